@@ -1,0 +1,163 @@
+"""Benchmark: Mrays/s of the Kerr geodesic renderer on one MI355X.
+
+Workload: 1920x1080, spin 0.998, inclination 85 deg, observer at 100 M,
+FOV 20 deg, DP5(4) tolerance 1e-9, up to 20000 steps per ray, disk from the
+ISCO to 30 M, FBM noise 6 octaves seed 1, bloom sigma 12 px, ACES + sRGB.
+One step = one full frame: trace (FP64) + bloom + display transform, with
+every buffer resident in HBM.  Rays are synthetic (pinhole camera), spp 1.
+
+Prints one JSON line.  `value` = rays per second over whole frames (Mrays/s);
+`trace_mrays_s` counts the tracer alone.  `roofline` prices the tracer's
+algorithmic FP64 FLOPs (exact per-unit counts from
+tests/golden/flop_constants.json times the work counters the kernel writes)
+against the FP64 vector peak; `post_roofline` prices bloom + display
+against HBM bandwidth.  `cpu_baseline` times the FP64 CPU oracle on a
+bounded row sample of the same frame on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_VECTOR = 78.6e12   # MI355X dense FP64 vector FLOP/s
+PEAK_FP32_VECTOR = 157.3e12  # MI355X FP32 vector FLOP/s
+PEAK_HBM = 8.0e12            # bytes/s
+
+
+def flop_constants():
+    with open(os.path.join(ROOT, "tests", "golden", "flop_constants.json")) as fh:
+        return json.load(fh)
+
+
+def trace_flops(st, fc):
+    return st["rhs_evals"] * fc["rhs"] + st["attempts"] * fc["attempt_overhead"] + st["accepted"] * fc["commit"]
+
+
+def cpu_baseline(cfg, cam, budget_s=20.0):
+    """FP64 oracle on a row sample of the frame, all host cores."""
+    from oracle import oracle as O
+    threads = os.cpu_count() or 1
+    # probe: a sparse row sample to estimate the per-row cost
+    step = max(1, cfg.height // 16)
+    _, _, _, st = O.render_rows(cfg, cam, (0, cfg.height, step), threads)
+    per_row = st.seconds / max(1, len(range(0, cfg.height, step)))
+    rows_budget = max(1, int(budget_s / max(per_row, 1e-6)))
+    step2 = max(1, -(-cfg.height // rows_budget))
+    t0 = time.time()
+    _, _, _, st2 = O.render_rows(cfg, cam, (0, cfg.height, step2), threads)
+    nrows = len(range(0, cfg.height, step2))
+    return {
+        "value": st2.rays / st2.seconds / 1e6,
+        "unit": "Mrays/s",
+        "cores": int(st2.threads),
+        "kind": "port",
+        "sample": f"every {step2}th row of the frame ({nrows} rows x {cfg.width} px = {st2.rays} rays), "
+                  f"FP64 oracle, {st2.threads} threads, {st2.seconds:.1f} s",
+        "seconds": round(time.time() - t0, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
+                 "multi-GPU runs are not part of this build (use --gpus 1)")
+
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.renderer import HipRenderer
+
+    torch.cuda.set_device(0)
+    cfg, cam = K.preset("hd1080_a0998")
+    W, H = cfg.width, cfg.height
+    rays = W * H * cfg.spp
+    fc = flop_constants()
+    r = HipRenderer(cfg)
+    hdr = r.alloc_hdr()
+    bloom = torch.empty_like(hdr)
+    ldr = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    for _ in range(args.warmup):
+        r.trace(cam, hdr=hdr, with_class=False)
+        r.post_process(hdr, ldr=ldr, bloom=bloom)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done ({args.warmup} frames)", file=sys.stderr, flush=True)
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for k in range(args.steps):
+        ev[k][0].record()
+        r.trace(cam, hdr=hdr, with_class=False)
+        ev[k][1].record()
+        r.post_process(hdr, ldr=ldr, bloom=bloom)
+        ev[k][2].record()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(stop)
+    trace_ms = [e[0].elapsed_time(e[1]) for e in ev]
+    post_ms = [e[1].elapsed_time(e[2]) for e in ev]
+    st = r.stats()
+    ms_per_step = total_ms / args.steps
+    t_trace = sum(trace_ms) / len(trace_ms) / 1e3
+    t_post = sum(post_ms) / len(post_ms) / 1e3
+    flops = trace_flops(st, fc)
+    achieved = flops / t_trace
+    post_bytes = 20.0 * W * H  # compulsory: float4 HDR read + u32 write per pixel
+    out = {
+        "metric": "Mrays/s at 1080p a=0.998: FP64 vs FP32-mixed; backward/forward time ratio",
+        "value": round(rays / (ms_per_step / 1e3) / 1e6, 4),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "1920x1080 a=0.998 i=85deg r_obs=100M fov=20deg tol=1e-9 FP64 trace + bloom(sigma 12) + ACES/sRGB",
+                   "rays": rays, "parallelism": "single GPU"},
+        "trace_mrays_s": round(rays / t_trace / 1e6, 4),
+        "trace_ms": round(t_trace * 1e3, 4),
+        "post_ms": round(t_post * 1e3, 4),
+        "wall_s": round(wall, 3),
+        "work": {"rhs_evals": st["rhs_evals"], "attempts": st["attempts"], "accepted": st["accepted"],
+                 "crossings": st["crossings"], "max_attempts_ray": st["max_attempts_ray"],
+                 "attempts_per_ray": round(st["attempts"] / rays, 3)},
+        "roofline": {"bound": "valu-fp64", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4), "traffic": None,
+                     "kernel": "trace_fp64_persistent",
+                     "flop_per_frame": flops},
+        "post_roofline": {"bound": "hbm", "achieved": round(post_bytes / t_post / 1e9, 2), "peak": PEAK_HBM / 1e9,
+                          "unit": "GB/s", "frac": round(post_bytes / t_post / PEAK_HBM, 4), "traffic": None,
+                          "kernels": "bloom_h + post_v"},
+    }
+    print(f"[bench] timed {args.steps} frames: {ms_per_step:.2f} ms/frame", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cfg, cam, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

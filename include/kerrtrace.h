@@ -1,0 +1,236 @@
+/*
+ * kerrtrace.h -- C ABI of the MI355X (gfx950) Kerr geodesic renderer.
+ *
+ * The reference (DamoyY/KerrTrace) has no C ABI of its own: its Rust host
+ * (`CudaRenderer`, src/renderer/mod.rs:52-293) drives cudarc directly.  The
+ * entry points below replace, one for one, what that host does, so a Rust
+ * (or any FFI) host binds this library instead of cudarc:
+ *
+ *   kt_config            <- Config tree, src/main.rs:14-122 (+ config.yaml:1-67)
+ *   kt_config_defaults   <- config.yaml:1-67 default values
+ *   kt_create            <- CudaRenderer::new, src/renderer/mod.rs:76-117
+ *   kt_submit_render     <- CudaRenderer::submit_render, mod.rs:119-218
+ *   kt_ready_frame       <- CudaRenderer::ready_frame, mod.rs:220-250
+ *   kt_finish_frame      <- CudaRenderer::finish_frame, mod.rs:252-260
+ *   kt_destroy           <- Drop of CudaRenderer (mod.rs:52-74 fields)
+ *   kt_trace             <- launch of trace_kernel, cuda/kernel.cu:88-142
+ *                           (mod.rs:153-173)
+ *   kt_post_process      <- launches of bloom_horizontal + post_process,
+ *                           cuda/kernel.cu:144-251 (mod.rs:177-206)
+ *
+ * Additions with no reference counterpart (precision modes, batched
+ * launch, adjoint) are marked as such below.
+ *
+ * Conventions: every function returns a kt_status (0 = ok, < 0 = error);
+ * no C++ exception or abort crosses the ABI.  The message of the last error
+ * is read with kt_last_error.  Device pointers are plain `void*`/typed
+ * pointers into HIP device memory; `stream` is a hipStream_t passed as
+ * `void*` (NULL = the null stream).  A renderer handle is owned by one
+ * thread and is not re-entrant (the reference's winit loop is single
+ * threaded, src/main.rs:205-206).
+ */
+#ifndef KERRTRACE_H
+#define KERRTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t kt_status;
+enum {
+    KT_OK = 0,
+    KT_ERR_INVALID_CONFIG = -1,   /* build_kerr_params / validate_bloom_settings errors */
+    KT_ERR_DEVICE = -2,           /* HIP runtime error (allocation, copy, sync) */
+    KT_ERR_LAUNCH = -3,           /* kernel launch failure */
+    KT_ERR_LUT_OVERFLOW = -4,     /* colour temperature above lut_max_temp (kernel.cu:3-10, mod.rs:236-241) */
+    KT_ERR_QUEUE_ORDER = -5,      /* finish_frame out of FIFO order (mod.rs:252-256) */
+    KT_ERR_INVALID_ARGUMENT = -6, /* null pointer, size mismatch, bad index */
+    KT_ERR_CAPACITY = -7          /* request larger than a fixed internal limit */
+};
+
+/* Precision of the tracer.  The reference traces in FP32 with fast math
+ * (config.yaml:40, src/renderer/init.rs:114); both modes here are new. */
+enum {
+    KT_PRECISION_FP64 = 0,  /* every operation of the ray in FP64 */
+    KT_PRECISION_MIXED = 1  /* RK stage bulk in FP32; r-solve, error norm,
+                               step controller, disk root, state and
+                               affine accumulators in FP64 */
+};
+
+/* Equatorial-crossing root (cuda/kerr_trace.cuh:138-174). */
+enum {
+    KT_DISK_ROOT_SECANT_REF = 0, /* as written: the Newton loop at lines 147-168
+                                    updates a shadowing local, so the crossing
+                                    parameter is the clamped secant y0/(y0-y1) */
+    KT_DISK_ROOT_TRUE = 1        /* safeguarded Newton/bisection on the DOPRI5
+                                    dense-output interpolant, in FP64 */
+};
+
+/* Per-pixel hit-class byte written by kt_trace (new; the reference only
+ * writes colour).  Bits 0-1: terminal reason; higher bits: flags. */
+enum {
+    KT_END_OTHER = 0,       /* step budget or transmittance cutoff exhausted,
+                               or the state became non-finite */
+    KT_END_HORIZON = 1,     /* r < r_h + horizon_epsilon (kerr_trace.cuh:135-137) */
+    KT_END_ESCAPE = 2,      /* r > escape_radius (kerr_trace.cuh:229-237) */
+    KT_FLAG_DISK = 4,       /* at least one disk emission was composited */
+    KT_FLAG_SKY_GRID = 8,   /* the escape direction fell on a sky grid line */
+    KT_FLAG_LUT_OVERFLOW = 16 /* a disk temperature exceeded lut_max_temp */
+};
+
+/*
+ * Renderer configuration.  Field for field a superset of the reference's
+ * Config tree (src/main.rs:14-122) restricted to what reaches the render
+ * path; floating-point knobs are double here (the reference stores f32).
+ * Knobs marked "ext" have no reference counterpart.
+ */
+typedef struct kt_config {
+    /* window: main.rs:27-31 */
+    uint32_t width;
+    uint32_t height;
+    /* kernel: main.rs:90-97 */
+    uint32_t spp;
+    uint32_t sky_grid_divisions;       /* kernel.sky.grid_divisions */
+    double exposure_scale;
+    double sky_line_thickness;
+    double sky_intensity;
+    double spin;                       /* kernel.black_hole.spin (a*M, dimensional) */
+    double mass;                       /* kernel.black_hole.mass */
+    double disk_outer_radius;          /* kernel.disk.outer_radius */
+    double disk_temperature_scale;     /* kernel.disk.temperature_scale */
+    double initial_step;               /* kernel.integrator.* : main.rs:114-122 */
+    double tolerance;
+    uint32_t max_steps;
+    uint32_t max_attempts;
+    double transmittance_cutoff;
+    double horizon_epsilon;
+    double escape_radius;
+    /* blackbody: main.rs:67-73 */
+    uint32_t lut_size;
+    uint32_t reserved0;
+    double lut_max_temp;
+    double wavelength_start;
+    double wavelength_end;
+    double wavelength_step;
+    /* disk_noise: main.rs:79-85 */
+    int32_t noise_enabled;
+    uint32_t noise_detail;
+    double noise_scale;
+    double noise_strength;
+    double noise_winding;
+    uint64_t noise_seed;               /* ext: lattice-hash seed; 0 == reference hash */
+    /* bloom: main.rs:56-60 */
+    int32_t bloom_enabled;
+    int32_t reserved1;
+    double bloom_intensity;
+    double bloom_radius;               /* sigma = radius / 3 (mod.rs:141) */
+    double bloom_threshold;            /* ext: bright-pass max(c - thr, 0); 0 == reference */
+    /* extensions */
+    double tol_floor;                  /* ext: tol(H) = tol * max(5 - 5 t, tol_floor); 0 == reference */
+    double disk_peak_temperature;      /* ext: > 0 sets temperature_scale = peak / max(disk LUT) */
+    double mdot_scale;                 /* ext: temperature_scale *= mdot_scale^(1/4); 1 == reference */
+    int32_t disk_root_mode;            /* ext: KT_DISK_ROOT_* */
+    int32_t precision;                 /* ext: KT_PRECISION_* */
+} kt_config;
+
+/* Pinhole camera in FP64.  The reference passes 12 f32 scalars plus
+ * fov_scale = tan(fov/2) (cuda/kernel.cu:88-97, src/app.rs:163). */
+typedef struct kt_camera {
+    double pos[3];
+    double fwd[3];
+    double rgt[3];
+    double up[3];
+    double fov_scale;
+} kt_camera;
+
+/* Host-derived quantities (src/renderer/utils.rs:53-114, 172-201). */
+typedef struct kt_derived {
+    double rh;            /* M + sqrt(M^2 - a^2) */
+    double disk_inner;    /* prograde ISCO of |a|/M */
+    double disk_outer;
+    double temperature_scale; /* after peak normalisation and mdot scaling */
+    double disk_lut_max;  /* max_i f^(1/4)[i] */
+    double bloom_sigma;
+    int32_t bloom_radius_int; /* ceil(bloom.radius) (init.rs:58-66) */
+    int32_t bloom_active;
+} kt_derived;
+
+/* Work counters of the last kt_trace (new; used for the VALU roofline). */
+typedef struct kt_trace_stats {
+    uint64_t rays;
+    uint64_t rhs_evals;     /* evaluations of the Hamiltonian right-hand side */
+    uint64_t attempts;      /* DP5(4) step attempts */
+    uint64_t accepted;      /* accepted (not forced) steps */
+    uint64_t crossings;     /* equatorial crossings handled */
+    uint64_t max_attempts_ray; /* largest attempt count of any single ray */
+    uint32_t lut_overflow;  /* 1 if a disk temperature exceeded lut_max_temp */
+    float overflow_temperature; /* the first such temperature (kernel.cu:5-8) */
+} kt_trace_stats;
+
+/* Tuning knobs of the persistent tracer (new).  Zero means "default". */
+typedef struct kt_launch_opts {
+    int32_t waves_per_cu;   /* resident waves per CU of the persistent grid */
+    int32_t refill_threshold; /* refill a wave once this many lanes are idle */
+    int32_t queue_chunk;    /* pixel tile edge used to order the work queue */
+    int32_t reserved;
+} kt_launch_opts;
+
+typedef struct kt_renderer kt_renderer;
+
+typedef struct kt_frame_view {
+    int32_t index;            /* FrameView.index (mod.rs:20-23) */
+    uint32_t width;
+    uint32_t height;
+    uint32_t reserved;
+    const uint32_t* data;     /* 0x00RRGGBB, borrowed until kt_finish_frame */
+} kt_frame_view;
+
+/* ---- configuration ---------------------------------------------------- */
+kt_status kt_config_defaults(kt_config* cfg);
+kt_status kt_validate_config(const kt_config* cfg, kt_derived* out);
+kt_status kt_camera_from_yaw_pitch(const double pos[3], double yaw_deg,
+                                   double pitch_deg, double fov_deg,
+                                   kt_camera* out); /* src/math.rs:3-14, app.rs:163 */
+kt_status kt_camera_observer(double r_obs, double inclination_deg,
+                             double fov_deg, kt_camera* out);
+kt_status kt_last_error(const kt_renderer* r, char* buf, size_t len);
+/* Host-side lookup tables of a config, without a device (for checks):
+ * disk: double[lut_size] = f^(1/4) (utils.rs:172-201);
+ * blackbody: double[lut_size*3] linear-sRGB (blackbody.rs:56-124). */
+kt_status kt_host_luts(const kt_config* cfg, double* disk_lut, double* bb_lut);
+
+/* ---- lifecycle and frame API (src/renderer/mod.rs) ------------------- */
+kt_status kt_create(const kt_config* cfg, kt_renderer** out);
+void kt_destroy(kt_renderer* r);
+kt_status kt_get_derived(const kt_renderer* r, kt_derived* out);
+kt_status kt_submit_render(kt_renderer* r, const float cam_pos[3],
+                           const float fwd[3], const float rgt[3],
+                           const float up[3], float fov_scale,
+                           int32_t* submitted);
+kt_status kt_submit_render_f64(kt_renderer* r, const kt_camera* cam,
+                               int32_t* submitted);
+kt_status kt_ready_frame(kt_renderer* r, kt_frame_view* view,
+                         int32_t* has_frame);
+kt_status kt_finish_frame(kt_renderer* r, int32_t index);
+
+/* ---- stream-ordered device API on caller-owned buffers --------------- */
+/* hdr: float4[W*H] (rgb, 1); hit_class: uint8[W*H] or NULL. */
+kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr,
+                   uint8_t* hit_class, void* stream);
+kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* opts);
+/* Synchronises the renderer stream and returns the last trace's counters. */
+kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out);
+/* bloom: float4[W*H] scratch; ldr: uint32[W*H] 0x00RRGGBB. */
+kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom,
+                          uint32_t* ldr, void* stream);
+/* Host copies of the lookup tables the device uses (for parity checks):
+ * disk: double[lut_size]; blackbody: double[lut_size*3] (rgb). */
+kt_status kt_get_luts(const kt_renderer* r, double* disk_lut, double* bb_lut);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KERRTRACE_H */

@@ -1,0 +1,766 @@
+// Host runtime of the MI355X renderer: configuration checks, the host-built
+// lookup tables, device buffers, the double-buffered frame API and the
+// C ABI declared in include/kerrtrace.h.
+//
+// Maps to the reference's Rust host:
+//   derive()             <- build_kerr_params (src/renderer/utils.rs:53-114),
+//                           validate_bloom_settings (src/renderer/init.rs:43-73)
+//   disk_lut()           <- generate_disk_temperature_lut (utils.rs:172-201)
+//   blackbody_lut()      <- generate_blackbody_lut (src/renderer/blackbody.rs:56-124)
+//   kt_create            <- CudaRenderer::new (src/renderer/mod.rs:76-117)
+//   kt_submit_render*    <- submit_render (mod.rs:119-218)
+//   kt_ready_frame       <- ready_frame (mod.rs:220-250)
+//   kt_finish_frame      <- finish_frame (mod.rs:252-260)
+// The reference's textures (src/renderer/texture.rs) become plain device
+// arrays read with explicit linear interpolation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/kerrtrace.h"
+#include "kt_internal.hpp"
+
+namespace kt {
+hipError_t launch_reset(DevCounters* c, uint32_t* error_flag, hipStream_t stream);
+hipError_t launch_trace_fp64(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,
+                             DevCounters* d_counters, int blocks, hipStream_t stream);
+hipError_t launch_trace_mixed(const RenderParams* d_params, int n_renders, uint64_t total_items,
+                              int refill_threshold, DevCounters* d_counters, int blocks, hipStream_t stream);
+hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, const float* d_wts, float threshold,
+                          int enabled, hipStream_t stream);
+hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int TH,
+                         const float* d_wts, float intensity, int bloom_on, hipStream_t stream);
+size_t bloom_h_lds_bytes(int R);
+size_t post_v_lds_bytes(int R, int TH);
+}  // namespace kt
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+thread_local std::string g_last_error;
+
+// ------------------------------------------------------------ host physics
+// Prograde/retrograde ISCO (utils.rs:11-20).
+double calc_isco(double a_norm, bool prograde, double mass) {
+    double aa = a_norm * a_norm;
+    double z1 = std::fma(std::cbrt(1.0 - aa), std::cbrt(1.0 + a_norm) + std::cbrt(1.0 - a_norm), 1.0);
+    double z2 = std::sqrt(std::fma(3.0, aa, z1 * z1));
+    double sign = prograde ? -1.0 : 1.0;
+    double term_inside = (3.0 - z1) * std::fma(2.0, z2, 3.0 + z1);
+    return mass * (3.0 + z2 + sign * std::sqrt(std::fmax(term_inside, 0.0)));
+}
+
+// Novikov-Thorne flux factor exactly as the reference writes it
+// (utils.rs:21-52): of the three cubic-root log terms only those whose
+// root denominator is positive are kept.
+double nt_factor(double r, double a_norm, double r_isco, double inv_m) {
+    if (r <= r_isco) return 0.0;
+    double r_norm = r * inv_m;
+    double x = std::sqrt(r_norm);
+    double x_ms = std::sqrt(r_isco * inv_m);
+    double base = std::acos(-a_norm) * (1.0 / 3.0);
+    const double step = 2.0 * kPi / 3.0;
+    const double roots[3] = {2.0 * std::cos(base), 2.0 * std::cos(base - step), 2.0 * std::cos(base + step)};
+    double sum_log = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double xi = roots[i];
+        double denom = xi * (xi - roots[(i + 1) % 3]) * (xi - roots[(i + 2) % 3]);
+        if (denom <= 0.0) continue;
+        double coef = 3.0 * (xi - a_norm) * (xi - a_norm) / denom;
+        double v = (x - xi) / (x_ms - xi);
+        if (v > 0.0) sum_log = sum_log + coef * std::log(v);
+    }
+    double q = std::fma(1.5 * a_norm, -std::log(x / x_ms), x - x_ms) - sum_log;
+    double geo = r_norm * std::fma(2.0, a_norm, std::fma(r_norm, x, -(3.0 * x)));
+    return std::fmax(q / geo, 0.0);
+}
+
+// Multi-lobe Gaussian fit of the CIE 1931 colour-matching functions
+// (Wyman, Sloan and Shirley 2013) as used by blackbody.rs:6-25.
+double cmf_lobe(double x, double alpha, double mu, double s1, double s2) {
+    double sigma = x < mu ? s1 : s2;
+    double t = (x - mu) / sigma;
+    return alpha * std::exp(-0.5 * t * t);
+}
+
+// Linear-sRGB colour of a blackbody for each LUT temperature
+// (blackbody.rs:26-124): Planck radiance (c2 = 1.4388e7 nm K, scaled by
+// 1e15, zero beyond exponent 80) times the CMFs, summed over the sampled
+// wavelengths, then the IEC 61966-2-1 XYZ -> linear sRGB matrix.
+void blackbody_lut(const kt_config& c, std::vector<double>& out) {
+    const int n = (int)c.lut_size;
+    out.assign((size_t)n * 3, 0.0);
+    std::vector<double> lam, cx, cy, cz;
+    for (double l = c.wavelength_start; !(l > c.wavelength_end); l += c.wavelength_step) {
+        lam.push_back(l);
+        cx.push_back(cmf_lobe(l, 1.056, 599.8, 37.9, 31.0) + cmf_lobe(l, 0.362, 442.0, 16.0, 26.7) +
+                     cmf_lobe(l, -0.065, 501.1, 20.4, 26.2));
+        cy.push_back(cmf_lobe(l, 0.821, 568.8, 46.9, 40.5) + cmf_lobe(l, 0.286, 530.9, 16.3, 31.1));
+        cz.push_back(cmf_lobe(l, 1.217, 437.0, 11.8, 36.0) + cmf_lobe(l, 0.681, 459.0, 26.2, 13.8));
+    }
+    const double c2 = 1.4388e7;
+    for (int i = 1; i < n; ++i) {
+        double t = ((double)i / (double)(n - 1)) * c.lut_max_temp;
+        if (t == 0.0) continue;
+        double X = 0.0, Y = 0.0, Z = 0.0;
+        for (size_t k = 0; k < lam.size(); ++k) {
+            double l = lam[k];
+            double e = c2 / (l * t);
+            double rad = 0.0;
+            if (!(e > 80.0)) {
+                double l5 = l * l * l * l * l;
+                rad = (1.0 / l5) / std::expm1(e) * 1e15;
+            }
+            X += rad * cx[k];
+            Y += rad * cy[k];
+            Z += rad * cz[k];
+        }
+        X *= c.wavelength_step;
+        Y *= c.wavelength_step;
+        Z *= c.wavelength_step;
+        double* o = &out[(size_t)i * 3];
+        o[0] = std::fma(3.240454, X, std::fma(-1.5371385, Y, -0.4985314 * Z));
+        o[1] = std::fma(-0.969266, X, std::fma(1.8760108, Y, 0.041556 * Z));
+        o[2] = std::fma(0.0556434, X, std::fma(-0.2040259, Y, 1.0572252 * Z));
+    }
+}
+
+struct Derived {
+    double rh, disk_inner, disk_outer, T_scale, lut_max;
+    std::vector<double> disk_lut, bb_lut;
+    int bloom_R;
+    double bloom_sigma;
+    bool bloom_active;
+};
+
+bool finite3(double a, double b, double c) { return std::isfinite(a) && std::isfinite(b) && std::isfinite(c); }
+
+// Validates a config and derives everything the kernels read.
+int derive(const kt_config& c, Derived& d, std::string& why) {
+    if (c.width == 0 || c.height == 0) { why = "window width and height must be non-zero"; return KT_ERR_INVALID_CONFIG; }
+    if (!std::isfinite(c.spin)) { why = "black hole spin is not finite"; return KT_ERR_INVALID_CONFIG; }
+    if (!std::isfinite(c.mass)) { why = "black hole mass is not finite"; return KT_ERR_INVALID_CONFIG; }
+    if (c.mass == 0.0) { why = "black hole mass must be non-zero"; return KT_ERR_INVALID_CONFIG; }
+    if (!finite3(c.noise_scale, c.noise_strength, c.noise_winding)) { why = "disk noise parameter is not finite"; return KT_ERR_INVALID_CONFIG; }
+    if (c.noise_detail > 0x7fffffffu) { why = "disk noise detail exceeds i32"; return KT_ERR_INVALID_CONFIG; }
+    if (c.lut_size < 2) { why = "lut_size must be at least 2"; return KT_ERR_INVALID_CONFIG; }
+    if (c.lut_size > (1u << 20)) { why = "lut_size above 2^20"; return KT_ERR_CAPACITY; }
+    if (!(std::isfinite(c.wavelength_step) && c.wavelength_step > 0.0)) { why = "wavelength_step must be positive and finite"; return KT_ERR_INVALID_CONFIG; }
+    if (!(std::isfinite(c.lut_max_temp) && c.lut_max_temp > 0.0)) { why = "lut_max_temp must be positive and finite"; return KT_ERR_INVALID_CONFIG; }
+    if (!(std::isfinite(c.bloom_intensity) && c.bloom_intensity >= 0.0)) { why = "bloom intensity invalid"; return KT_ERR_INVALID_CONFIG; }
+    if (!(std::isfinite(c.bloom_radius) && c.bloom_radius >= 0.0)) { why = "bloom radius invalid"; return KT_ERR_INVALID_CONFIG; }
+    if (c.bloom_radius > 2147483647.0) { why = "bloom radius too large"; return KT_ERR_INVALID_CONFIG; }
+    if (!(std::isfinite(c.bloom_threshold) && c.bloom_threshold >= 0.0)) { why = "bloom threshold invalid"; return KT_ERR_INVALID_CONFIG; }
+    if (c.spp == 0) { why = "spp must be positive"; return KT_ERR_INVALID_CONFIG; }
+    if (c.max_attempts == 0) { why = "max_attempts must be positive"; return KT_ERR_INVALID_CONFIG; }
+    if (c.sky_grid_divisions == 0) { why = "sky grid divisions must be positive"; return KT_ERR_INVALID_CONFIG; }
+    if (!(std::isfinite(c.mdot_scale) && c.mdot_scale >= 0.0)) { why = "mdot_scale must be finite and non-negative"; return KT_ERR_INVALID_CONFIG; }
+    if (c.precision != KT_PRECISION_FP64 && c.precision != KT_PRECISION_MIXED) { why = "unknown precision mode"; return KT_ERR_INVALID_CONFIG; }
+    if (c.disk_root_mode != KT_DISK_ROOT_SECANT_REF && c.disk_root_mode != KT_DISK_ROOT_TRUE) { why = "unknown disk root mode"; return KT_ERR_INVALID_CONFIG; }
+    if ((uint64_t)c.width * c.height > (1ull << 31)) { why = "frame larger than 2^31 pixels"; return KT_ERR_CAPACITY; }
+    const double M = c.mass;
+    const double inv_m = 1.0 / M;
+    const double aa = c.spin * c.spin;
+    const double a_norm = c.spin * inv_m;
+    d.rh = M + std::sqrt(std::fmax(std::fma(M, M, -aa), 0.0));
+    d.disk_inner = calc_isco(a_norm, true, M);
+    if (!std::isfinite(d.rh) || !std::isfinite(d.disk_inner)) { why = "Kerr parameters are not finite"; return KT_ERR_INVALID_CONFIG; }
+    d.disk_outer = c.disk_outer_radius;
+    if (!std::isfinite(d.disk_outer)) { why = "disk outer radius is not finite"; return KT_ERR_INVALID_CONFIG; }
+    if (!(d.disk_outer > d.disk_inner)) {
+        char b[160];
+        std::snprintf(b, sizeof b, "disk outer radius must exceed the inner radius: outer=%g, inner=%g", d.disk_outer, d.disk_inner);
+        why = b;
+        return KT_ERR_INVALID_CONFIG;
+    }
+    const int n = (int)c.lut_size;
+    d.disk_lut.resize(n);
+    const double span = d.disk_outer - d.disk_inner;
+    d.lut_max = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double r = std::fma((double)i / (double)(n - 1), span, d.disk_inner);
+        double f = nt_factor(r, a_norm, d.disk_inner, inv_m);
+        if (!std::isfinite(f)) { why = "disk flux factor is not finite"; return KT_ERR_INVALID_CONFIG; }
+        double v = std::sqrt(std::sqrt(f));
+        d.disk_lut[i] = v;
+        d.lut_max = std::max(d.lut_max, v);
+    }
+    double T_scale = c.disk_temperature_scale;
+    if (c.disk_peak_temperature > 0.0) {
+        if (!(d.lut_max > 0.0)) { why = "disk flux LUT is identically zero"; return KT_ERR_INVALID_CONFIG; }
+        T_scale = c.disk_peak_temperature / d.lut_max;
+    }
+    d.T_scale = T_scale * std::sqrt(std::sqrt(c.mdot_scale));
+    blackbody_lut(c, d.bb_lut);
+    d.bloom_R = c.bloom_radius > 0.0 ? (int)std::ceil(c.bloom_radius) : 0;
+    d.bloom_sigma = c.bloom_radius / 3.0;
+    d.bloom_active = c.bloom_enabled != 0 && d.bloom_R > 0 && c.bloom_intensity > 0.0;
+    return KT_OK;
+}
+
+void fill_derived(const Derived& d, kt_derived* out) {
+    out->rh = d.rh;
+    out->disk_inner = d.disk_inner;
+    out->disk_outer = d.disk_outer;
+    out->temperature_scale = d.T_scale;
+    out->disk_lut_max = d.lut_max;
+    out->bloom_sigma = d.bloom_sigma;
+    out->bloom_radius_int = d.bloom_R;
+    out->bloom_active = d.bloom_active ? 1 : 0;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- renderer
+struct kt_renderer {
+    kt_config cfg;
+    Derived d;
+    std::string err;
+    int device = 0;
+    int cus = 256;
+    size_t max_lds = 65536;
+    hipStream_t stream = nullptr;
+    double* d_disk_lut = nullptr;
+    double* d_bb_lut = nullptr;
+    float* d_disk_lut_f = nullptr;
+    float* d_bb_lut_f = nullptr;
+    float* d_bloom_w = nullptr;
+    int post_th = 64;
+    kt::DevCounters* d_counters = nullptr;
+    uint32_t* d_trace_flag = nullptr;
+    // ring of parameter records: pinned host staging -> device
+    static constexpr int kRing = 16;
+    kt::RenderParams* h_params = nullptr;
+    kt::RenderParams* d_params = nullptr;
+    hipEvent_t param_ev[kRing] = {};
+    bool param_ev_used[kRing] = {};
+    int param_next = 0;
+    kt_launch_opts opts{8, 8, 8, 0};
+    hipStream_t last_stream = nullptr;
+    uint64_t last_rays = 0;
+    // double-buffered frames (mod.rs:24-51, 88-91)
+    struct Frame {
+        float* hdr = nullptr;
+        float* bloom = nullptr;
+        uint32_t* ldr = nullptr;
+        uint32_t* ldr_host = nullptr;
+        uint32_t* flag = nullptr;
+        uint32_t* flag_host = nullptr;
+        hipEvent_t ready = nullptr;
+        bool pending = false;
+    };
+    Frame frames[2];
+    std::deque<int> queue;
+    int next_frame = 0;
+};
+
+namespace {
+
+int set_err(kt_renderer* r, int code, const std::string& msg) {
+    if (r) r->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+int hip_err(kt_renderer* r, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_err(r, KT_ERR_DEVICE, m);
+}
+
+#define KT_HIP(r, call, what)                      \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return hip_err(r, e_, what); \
+    } while (0)
+
+void release(kt_renderer* r) {
+    if (!r) return;
+    for (auto& f : r->frames) {
+        if (f.hdr) (void)hipFree(f.hdr);
+        if (f.bloom) (void)hipFree(f.bloom);
+        if (f.ldr) (void)hipFree(f.ldr);
+        if (f.flag) (void)hipFree(f.flag);
+        if (f.ldr_host) (void)hipHostFree(f.ldr_host);
+        if (f.flag_host) (void)hipHostFree(f.flag_host);
+        if (f.ready) (void)hipEventDestroy(f.ready);
+    }
+    for (int i = 0; i < kt_renderer::kRing; ++i)
+        if (r->param_ev[i]) (void)hipEventDestroy(r->param_ev[i]);
+    if (r->h_params) (void)hipHostFree(r->h_params);
+    if (r->d_params) (void)hipFree(r->d_params);
+    if (r->d_disk_lut) (void)hipFree(r->d_disk_lut);
+    if (r->d_bb_lut) (void)hipFree(r->d_bb_lut);
+    if (r->d_disk_lut_f) (void)hipFree(r->d_disk_lut_f);
+    if (r->d_bb_lut_f) (void)hipFree(r->d_bb_lut_f);
+    if (r->d_bloom_w) (void)hipFree(r->d_bloom_w);
+    if (r->d_counters) (void)hipFree(r->d_counters);
+    if (r->d_trace_flag) (void)hipFree(r->d_trace_flag);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+// The parameter record of one render with camera `cam` writing to hdr/cls.
+void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, const kt_camera& cam, float* hdr,
+                 uint8_t* cls, uint32_t* flag, int tile, kt::RenderParams& P) {
+    std::memset(&P, 0, sizeof P);
+    P.a = c.spin;
+    P.M = c.mass;
+    P.aa = c.spin * c.spin;
+    P.inv_M = 1.0 / c.mass;
+    P.rh_eps = d.rh + c.horizon_epsilon;
+    P.disk_inner = d.disk_inner;
+    P.disk_outer = d.disk_outer;
+    P.T_scale = d.T_scale;
+    P.h0 = c.initial_step;
+    P.tol = c.tolerance;
+    P.tol_floor = c.tol_floor;
+    P.cutoff = c.transmittance_cutoff;
+    P.R_esc = c.escape_radius;
+    P.inv_w_2 = 2.0 / (double)c.width;
+    P.inv_h_2 = 2.0 / (double)c.height;
+    P.aspect = (double)c.width / (double)c.height;
+    P.exposure_over_spp = c.exposure_scale * (1.0 / (double)c.spp);
+    for (int k = 0; k < 3; ++k) {
+        P.cam[k] = cam.pos[k];
+        P.fwd[k] = cam.fwd[k];
+        P.rgt[k] = cam.rgt[k];
+        P.up[k] = cam.up[k];
+    }
+    P.fov_scale = cam.fov_scale;
+    P.sky_spacing = kPi / (double)c.sky_grid_divisions;
+    P.sky_lt = c.sky_line_thickness;
+    P.sky_int = c.sky_intensity;
+    P.noise_scale = c.noise_scale;
+    P.noise_strength = c.noise_strength;
+    P.noise_winding = c.noise_winding;
+    P.noise_seed_mix = c.noise_seed * 0x9E3779B97F4A7C15ULL;
+    P.wl_step = c.wavelength_step;
+    P.lut_max_temp = c.lut_max_temp;
+    P.disk_lut = r ? r->d_disk_lut : nullptr;
+    P.bb_lut = r ? r->d_bb_lut : nullptr;
+    P.disk_lut_f = r ? r->d_disk_lut_f : nullptr;
+    P.bb_lut_f = r ? r->d_bb_lut_f : nullptr;
+    P.hdr = hdr;
+    P.cls = cls;
+    P.error_flag = flag;
+    P.W = (int)c.width;
+    P.H = (int)c.height;
+    P.spp = (int)c.spp;
+    P.max_steps = (int)std::min<uint32_t>(c.max_steps, 0x7fffffffu);
+    P.max_attempts = (int)std::min<uint32_t>(c.max_attempts, 0x7fffffffu);
+    P.lut_n = (int)c.lut_size;
+    bool noise_on = c.noise_enabled != 0 && c.noise_detail > 0 && c.noise_strength != 0.0;
+    P.noise_detail = noise_on ? (int)c.noise_detail : 0;
+    P.disk_root_mode = c.disk_root_mode;
+    P.tile = tile;
+    P.tiles_x = ((int)c.width + tile - 1) / tile;
+    int tiles_y = ((int)c.height + tile - 1) / tile;
+    P.ray_begin = 0;
+    P.ray_count = (uint64_t)P.tiles_x * (uint64_t)tiles_y * (uint64_t)tile * (uint64_t)tile;
+}
+
+// Stages a parameter record through the pinned ring and returns its
+// device address (stream-ordered).
+int upload_params(kt_renderer* r, const kt::RenderParams& P, hipStream_t s, kt::RenderParams** out) {
+    int i = r->param_next;
+    r->param_next = (i + 1) % kt_renderer::kRing;
+    if (r->param_ev_used[i]) KT_HIP(r, hipEventSynchronize(r->param_ev[i]), "param ring wait");
+    r->h_params[i] = P;
+    KT_HIP(r, hipMemcpyAsync(&r->d_params[i], &r->h_params[i], sizeof P, hipMemcpyHostToDevice, s), "param upload");
+    KT_HIP(r, hipEventRecord(r->param_ev[i], s), "param event");
+    r->param_ev_used[i] = true;
+    *out = &r->d_params[i];
+    return KT_OK;
+}
+
+int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* flag, hipStream_t s) {
+    kt::RenderParams P;
+    int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
+    make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P);
+    kt::RenderParams* dp = nullptr;
+    int rc = upload_params(r, P, s, &dp);
+    if (rc != KT_OK) return rc;
+    KT_HIP(r, kt::launch_reset(r->d_counters, flag, s), "counter reset launch");
+    int wpc = r->opts.waves_per_cu > 0 ? r->opts.waves_per_cu : 8;
+    int blocks = std::max(1, (r->cus * wpc + 3) / 4);
+    int thr = r->opts.refill_threshold > 0 ? std::min(r->opts.refill_threshold, 64) : 8;
+    hipError_t e = r->cfg.precision == KT_PRECISION_MIXED
+                       ? kt::launch_trace_mixed(dp, 1, P.ray_count, thr, r->d_counters, blocks, s)
+                       : kt::launch_trace_fp64(dp, 1, P.ray_count, thr, r->d_counters, blocks, s);
+    if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
+    r->last_stream = s;
+    r->last_rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
+    return KT_OK;
+}
+
+int post_into(kt_renderer* r, const float* hdr, float* bloom, uint32_t* ldr, hipStream_t s) {
+    const kt_config& c = r->cfg;
+    int W = (int)c.width, H = (int)c.height, R = r->d.bloom_R;
+    bool on = r->d.bloom_active;
+    if (on) {
+        hipError_t e = kt::launch_bloom_h(hdr, bloom, W, H, R, r->d_bloom_w, (float)c.bloom_threshold, 1, s);
+        if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("bloom launch: ") + hipGetErrorString(e));
+    }
+    hipError_t e = kt::launch_post_v(hdr, bloom, ldr, W, H, R, r->post_th, r->d_bloom_w, (float)c.bloom_intensity,
+                                     on ? 1 : 0, s);
+    if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("post launch: ") + hipGetErrorString(e));
+    return KT_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+kt_status kt_config_defaults(kt_config* c) {
+    if (!c) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null config");
+    std::memset(c, 0, sizeof *c);
+    c->width = 2100;
+    c->height = 900;
+    c->spp = 4;
+    c->sky_grid_divisions = 40;
+    c->exposure_scale = 3e-3;
+    c->sky_line_thickness = 5e-4;
+    c->sky_intensity = 30.0;
+    c->spin = -0.45;
+    c->mass = 0.5;
+    c->disk_outer_radius = 40.0;
+    c->disk_temperature_scale = 3e4;
+    c->initial_step = 1.0;
+    c->tolerance = 1e-5;
+    c->max_steps = 100;
+    c->max_attempts = 3;
+    c->transmittance_cutoff = 1e-2;
+    c->horizon_epsilon = 5e-7;
+    c->escape_radius = 80.0;
+    c->lut_size = 4096;
+    c->lut_max_temp = 3e4;
+    c->wavelength_start = 380.0;
+    c->wavelength_end = 720.0;
+    c->wavelength_step = 10.0;
+    c->noise_enabled = 1;
+    c->noise_detail = 10;
+    c->noise_scale = 0.1;
+    c->noise_strength = 0.6;
+    c->noise_winding = 50.0;
+    c->noise_seed = 0;
+    c->bloom_enabled = 1;
+    c->bloom_intensity = 0.5;
+    c->bloom_radius = 48.0;
+    c->bloom_threshold = 0.0;
+    c->tol_floor = 0.0;
+    c->disk_peak_temperature = 0.0;
+    c->mdot_scale = 1.0;
+    c->disk_root_mode = KT_DISK_ROOT_SECANT_REF;
+    c->precision = KT_PRECISION_FP64;
+    return KT_OK;
+}
+
+kt_status kt_validate_config(const kt_config* c, kt_derived* out) {
+    if (!c) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null config");
+    Derived d;
+    std::string why;
+    int rc = derive(*c, d, why);
+    if (rc != KT_OK) return set_err(nullptr, rc, why);
+    if (out) fill_derived(d, out);
+    return KT_OK;
+}
+
+kt_status kt_host_luts(const kt_config* c, double* disk, double* bb) {
+    if (!c) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null config");
+    Derived d;
+    std::string why;
+    int rc = derive(*c, d, why);
+    if (rc != KT_OK) return set_err(nullptr, rc, why);
+    if (disk) std::memcpy(disk, d.disk_lut.data(), d.disk_lut.size() * sizeof(double));
+    if (bb) std::memcpy(bb, d.bb_lut.data(), d.bb_lut.size() * sizeof(double));
+    return KT_OK;
+}
+
+kt_status kt_camera_from_yaw_pitch(const double pos[3], double yaw_deg, double pitch_deg, double fov_deg,
+                                   kt_camera* out) {
+    if (!pos || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    double yaw = yaw_deg * (kPi / 180.0), pitch = pitch_deg * (kPi / 180.0);
+    double f[3] = {std::sin(yaw) * std::cos(pitch), std::sin(pitch), -std::cos(yaw) * std::cos(pitch)};
+    auto norm = [](double v[3]) {
+        double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] /= n; v[1] /= n; v[2] /= n;
+    };
+    norm(f);
+    double rgt[3] = {f[1] * 0.0 - f[2] * 1.0, f[2] * 0.0 - f[0] * 0.0, f[0] * 1.0 - f[1] * 0.0};
+    norm(rgt);
+    double up[3] = {rgt[1] * f[2] - rgt[2] * f[1], rgt[2] * f[0] - rgt[0] * f[2], rgt[0] * f[1] - rgt[1] * f[0]};
+    norm(up);
+    for (int k = 0; k < 3; ++k) {
+        out->pos[k] = pos[k];
+        out->fwd[k] = f[k];
+        out->rgt[k] = rgt[k];
+        out->up[k] = up[k];
+    }
+    out->fov_scale = std::tan(fov_deg * (kPi / 180.0) / 2.0);
+    return KT_OK;
+}
+
+kt_status kt_camera_observer(double r_obs, double incl_deg, double fov_deg, kt_camera* out) {
+    if (!out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null camera");
+    if (!(std::isfinite(r_obs) && r_obs > 0.0 && std::isfinite(incl_deg) && std::isfinite(fov_deg)))
+        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "observer parameters must be finite, r_obs > 0");
+    double i = incl_deg * (kPi / 180.0);
+    double si = std::sin(i), ci = std::cos(i);
+    double pos[3] = {r_obs * si, r_obs * ci, 0.0};
+    double fwd[3] = {-si, -ci, 0.0};
+    double rgt[3] = {0.0, 0.0, -1.0};
+    double up[3] = {-ci, si, 0.0};
+    for (int k = 0; k < 3; ++k) {
+        out->pos[k] = pos[k];
+        out->fwd[k] = fwd[k];
+        out->rgt[k] = rgt[k];
+        out->up[k] = up[k];
+    }
+    out->fov_scale = std::tan(fov_deg * (kPi / 180.0) / 2.0);
+    return KT_OK;
+}
+
+kt_status kt_last_error(const kt_renderer* r, char* buf, size_t len) {
+    if (!buf || len == 0) return KT_ERR_INVALID_ARGUMENT;
+    const std::string& m = (r && !r->err.empty()) ? r->err : g_last_error;
+    std::snprintf(buf, len, "%s", m.c_str());
+    return KT_OK;
+}
+
+kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
+    if (!cfg || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    kt_renderer* r = new (std::nothrow) kt_renderer();
+    if (!r) return set_err(nullptr, KT_ERR_DEVICE, "out of host memory");
+    r->cfg = *cfg;
+    std::string why;
+    int rc = derive(r->cfg, r->d, why);
+    if (rc != KT_OK) {
+        release(r);
+        return set_err(nullptr, rc, why);
+    }
+#define KT_CREATE_HIP(call, what)                 \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess) {                   \
+            int rc_ = hip_err(nullptr, e_, what); \
+            release(r);                           \
+            return rc_;                           \
+        }                                         \
+    } while (0)
+    KT_CREATE_HIP(hipGetDevice(&r->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    KT_CREATE_HIP(hipGetDeviceProperties(&prop, r->device), "hipGetDeviceProperties");
+    r->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    r->max_lds = prop.sharedMemPerBlock > 0 ? prop.sharedMemPerBlock : 65536;
+    KT_CREATE_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const int n = (int)cfg->lut_size;
+    std::vector<float> disk_f(n), bb_f((size_t)n * 3);
+    for (int i = 0; i < n; ++i) disk_f[i] = (float)r->d.disk_lut[i];
+    for (size_t i = 0; i < bb_f.size(); ++i) bb_f[i] = (float)r->d.bb_lut[i];
+    KT_CREATE_HIP(hipMalloc(&r->d_disk_lut, sizeof(double) * n), "hipMalloc disk LUT");
+    KT_CREATE_HIP(hipMalloc(&r->d_bb_lut, sizeof(double) * n * 3), "hipMalloc blackbody LUT");
+    KT_CREATE_HIP(hipMalloc(&r->d_disk_lut_f, sizeof(float) * n), "hipMalloc disk LUT f32");
+    KT_CREATE_HIP(hipMalloc(&r->d_bb_lut_f, sizeof(float) * n * 3), "hipMalloc blackbody LUT f32");
+    KT_CREATE_HIP(hipMemcpy(r->d_disk_lut, r->d.disk_lut.data(), sizeof(double) * n, hipMemcpyHostToDevice), "upload disk LUT");
+    KT_CREATE_HIP(hipMemcpy(r->d_bb_lut, r->d.bb_lut.data(), sizeof(double) * n * 3, hipMemcpyHostToDevice), "upload blackbody LUT");
+    KT_CREATE_HIP(hipMemcpy(r->d_disk_lut_f, disk_f.data(), sizeof(float) * n, hipMemcpyHostToDevice), "upload disk LUT f32");
+    KT_CREATE_HIP(hipMemcpy(r->d_bb_lut_f, bb_f.data(), sizeof(float) * n * 3, hipMemcpyHostToDevice), "upload blackbody LUT f32");
+    // bloom tap weights exp(-i^2 / (2 sigma^2)) (kernel.cu:48-53)
+    const int R = r->d.bloom_R;
+    if (r->d.bloom_active) {
+        if (kt::bloom_h_lds_bytes(R) > r->max_lds) {
+            release(r);
+            return set_err(nullptr, KT_ERR_CAPACITY, "bloom radius too large for the LDS-staged blur");
+        }
+        int th = 64;
+        while (th > 4 && kt::post_v_lds_bytes(R, th) > r->max_lds) th /= 2;
+        if (kt::post_v_lds_bytes(R, th) > r->max_lds) {
+            release(r);
+            return set_err(nullptr, KT_ERR_CAPACITY, "bloom radius too large for the LDS-staged blur");
+        }
+        r->post_th = th;
+    }
+    std::vector<float> w((size_t)2 * R + 1, 0.f);
+    for (int i = -R; i <= R; ++i) {
+        double t = (double)i;
+        w[i + R] = (float)std::exp(-(t * t) / (2.0 * r->d.bloom_sigma * r->d.bloom_sigma));
+    }
+    KT_CREATE_HIP(hipMalloc(&r->d_bloom_w, sizeof(float) * w.size()), "hipMalloc bloom weights");
+    KT_CREATE_HIP(hipMemcpy(r->d_bloom_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice), "upload bloom weights");
+    KT_CREATE_HIP(hipMalloc(&r->d_counters, sizeof(kt::DevCounters)), "hipMalloc counters");
+    KT_CREATE_HIP(hipMemset(r->d_counters, 0, sizeof(kt::DevCounters)), "clear counters");
+    KT_CREATE_HIP(hipMalloc(&r->d_trace_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
+    KT_CREATE_HIP(hipMemset(r->d_trace_flag, 0, 2 * sizeof(uint32_t)), "clear flag");
+    KT_CREATE_HIP(hipHostMalloc(&r->h_params, sizeof(kt::RenderParams) * kt_renderer::kRing, hipHostMallocDefault), "pinned params");
+    KT_CREATE_HIP(hipMalloc(&r->d_params, sizeof(kt::RenderParams) * kt_renderer::kRing), "hipMalloc params");
+    for (int i = 0; i < kt_renderer::kRing; ++i)
+        KT_CREATE_HIP(hipEventCreateWithFlags(&r->param_ev[i], hipEventDisableTiming), "param event");
+    // frame buffers (init.rs:142-178), allocated for both frames
+    const size_t px = (size_t)cfg->width * cfg->height;
+    for (auto& f : r->frames) {
+        KT_CREATE_HIP(hipMalloc(&f.hdr, px * 4 * sizeof(float)), "hipMalloc HDR");
+        KT_CREATE_HIP(hipMalloc(&f.bloom, px * 4 * sizeof(float)), "hipMalloc bloom");
+        KT_CREATE_HIP(hipMalloc(&f.ldr, px * sizeof(uint32_t)), "hipMalloc image");
+        KT_CREATE_HIP(hipMalloc(&f.flag, 2 * sizeof(uint32_t)), "hipMalloc LUT flag");
+        KT_CREATE_HIP(hipHostMalloc(&f.ldr_host, px * sizeof(uint32_t), hipHostMallocDefault), "pinned image");
+        KT_CREATE_HIP(hipHostMalloc(&f.flag_host, 2 * sizeof(uint32_t), hipHostMallocDefault), "pinned flag");
+        f.flag_host[0] = f.flag_host[1] = 0;
+        KT_CREATE_HIP(hipEventCreateWithFlags(&f.ready, hipEventDisableTiming), "frame event");
+    }
+#undef KT_CREATE_HIP
+    *out = r;
+    return KT_OK;
+}
+
+void kt_destroy(kt_renderer* r) {
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->last_stream && r->last_stream != r->stream) (void)hipStreamSynchronize(r->last_stream);
+    release(r);
+}
+
+kt_status kt_get_derived(const kt_renderer* r, kt_derived* out) {
+    if (!r || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    fill_derived(r->d, out);
+    return KT_OK;
+}
+
+kt_status kt_get_luts(const kt_renderer* r, double* disk, double* bb) {
+    if (!r) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null renderer");
+    if (disk) std::memcpy(disk, r->d.disk_lut.data(), r->d.disk_lut.size() * sizeof(double));
+    if (bb) std::memcpy(bb, r->d.bb_lut.data(), r->d.bb_lut.size() * sizeof(double));
+    return KT_OK;
+}
+
+kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* o) {
+    if (!r || !o) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
+        o->queue_chunk < 0 || o->queue_chunk > 256)
+        return set_err(r, KT_ERR_INVALID_ARGUMENT, "launch option out of range");
+    r->opts = *o;
+    return KT_OK;
+}
+
+kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, void* stream) {
+    if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    return trace_into(r, cam, hdr, cls, r->d_trace_flag, (hipStream_t)stream);
+}
+
+kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
+    if (!r || !out) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    KT_HIP(r, hipStreamSynchronize(r->last_stream), "stream sync");
+    kt::DevCounters c;
+    uint32_t flag[2];
+    KT_HIP(r, hipMemcpy(&c, r->d_counters, sizeof c, hipMemcpyDeviceToHost), "read counters");
+    KT_HIP(r, hipMemcpy(flag, r->d_trace_flag, sizeof flag, hipMemcpyDeviceToHost), "read flag");
+    out->rays = r->last_rays;
+    out->rhs_evals = c.rhs_evals;
+    out->attempts = c.attempts;
+    out->accepted = c.accepted;
+    out->crossings = c.crossings;
+    out->max_attempts_ray = c.max_attempts_ray;
+    out->lut_overflow = flag[0];
+    float T;
+    std::memcpy(&T, &flag[1], sizeof T);
+    out->overflow_temperature = flag[0] ? T : 0.0f;
+    return KT_OK;
+}
+
+kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom, uint32_t* ldr, void* stream) {
+    if (!r || !hdr || !ldr || (r->d.bloom_active && !bloom)) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    return post_into(r, hdr, bloom, ldr, (hipStream_t)stream);
+}
+
+kt_status kt_submit_render_f64(kt_renderer* r, const kt_camera* cam, int32_t* submitted) {
+    if (!r || !cam || !submitted) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    *submitted = 0;
+    if (r->queue.size() == 2) return KT_OK;  // both frames in flight (mod.rs:127-129)
+    int idx = -1;
+    for (int off = 0; off < 2; ++off) {
+        int i = (r->next_frame + off) % 2;
+        if (!r->frames[i].pending) { idx = i; break; }
+    }
+    if (idx < 0) return set_err(r, KT_ERR_DEVICE, "no free frame buffer");
+    auto& f = r->frames[idx];
+    hipStream_t s = r->stream;
+    int rc = trace_into(r, cam, f.hdr, nullptr, f.flag, s);  // clears f.flag first
+    if (rc != KT_OK) return rc;
+    KT_HIP(r, hipMemcpyAsync(f.flag_host, f.flag, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "flag copy");
+    rc = post_into(r, f.hdr, f.bloom, f.ldr, s);
+    if (rc != KT_OK) return rc;
+    size_t px = (size_t)r->cfg.width * r->cfg.height;
+    KT_HIP(r, hipMemcpyAsync(f.ldr_host, f.ldr, px * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "image copy");
+    KT_HIP(r, hipEventRecord(f.ready, s), "frame event");
+    f.pending = true;
+    r->queue.push_back(idx);
+    r->next_frame = (idx + 1) % 2;
+    *submitted = 1;
+    return KT_OK;
+}
+
+kt_status kt_submit_render(kt_renderer* r, const float cam_pos[3], const float fwd[3], const float rgt[3],
+                           const float up[3], float fov_scale, int32_t* submitted) {
+    if (!cam_pos || !fwd || !rgt || !up) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    kt_camera c;
+    for (int k = 0; k < 3; ++k) {
+        c.pos[k] = cam_pos[k];
+        c.fwd[k] = fwd[k];
+        c.rgt[k] = rgt[k];
+        c.up[k] = up[k];
+    }
+    c.fov_scale = fov_scale;
+    return kt_submit_render_f64(r, &c, submitted);
+}
+
+kt_status kt_ready_frame(kt_renderer* r, kt_frame_view* view, int32_t* has_frame) {
+    if (!r || !view || !has_frame) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    *has_frame = 0;
+    if (r->queue.empty()) return KT_OK;
+    int idx = r->queue.front();
+    auto& f = r->frames[idx];
+    hipError_t q = hipEventQuery(f.ready);
+    if (q == hipErrorNotReady) return KT_OK;
+    if (q != hipSuccess) return hip_err(r, q, "frame event query");
+    if (f.flag_host[0] != 0) {
+        // the frame stays queued as in mod.rs:236-241; its index is reported
+        // so the caller can still release it with kt_finish_frame
+        view->index = idx;
+        view->data = nullptr;
+        float T;
+        std::memcpy(&T, &f.flag_host[1], sizeof T);
+        char b[160];
+        std::snprintf(b, sizeof b, "colour temperature exceeds lut_max_temp: %g > %g", (double)T, r->cfg.lut_max_temp);
+        return set_err(r, KT_ERR_LUT_OVERFLOW, b);
+    }
+    view->index = idx;
+    view->width = r->cfg.width;
+    view->height = r->cfg.height;
+    view->reserved = 0;
+    view->data = f.ldr_host;
+    *has_frame = 1;
+    return KT_OK;
+}
+
+kt_status kt_finish_frame(kt_renderer* r, int32_t index) {
+    if (!r) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null renderer");
+    if (r->queue.empty()) return set_err(r, KT_ERR_QUEUE_ORDER, "frame queue is empty");
+    int expect = r->queue.front();
+    if (expect != index) {
+        char b[96];
+        std::snprintf(b, sizeof b, "frame queue order: expect=%d got=%d", expect, index);
+        return set_err(r, KT_ERR_QUEUE_ORDER, b);
+    }
+    r->queue.pop_front();
+    r->frames[index].pending = false;
+    return KT_OK;
+}
+
+}  // extern "C"

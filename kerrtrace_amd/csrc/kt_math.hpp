@@ -1,0 +1,262 @@
+// Device math of the FP64 tracer (gfx950).
+//
+// Every trajectory operation is an IEEE-exact binary64 +, -, *, /, sqrt or
+// fma, in a fixed order (the library is built with -ffp-contract=off, so
+// a*b+c is never fused behind the source's back); the only non-elementary
+// trajectory function is pow_m02, written below from its mathematics.  The
+// tracer therefore reproduces any binary64 evaluation of the same operation
+// sequence bit for bit.  Shading-only transcendentals (exp, sin, cos, atan2,
+// asin) come from the device math library and differ from a host libm by at
+// most a few ulp; they never feed back into a trajectory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_internal.hpp"
+
+namespace kt {
+
+// cuda/kerr_math.cuh:26-32: one 64-bit LCG step then the XSH-RR output.
+__device__ __forceinline__ uint32_t pcg32(uint64_t state) {
+    uint64_t x = state * 6364136223846793005ULL + 1442695040888963407ULL;
+    uint32_t xs = (uint32_t)(((x >> 18u) ^ x) >> 27u);
+    uint32_t rot = (uint32_t)(x >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+// x^(-1/5) for the step-size controller (kerr_trace.cuh:119 uses __powf).
+// x = m 2^e, m in [1,2), e = 5q + j: x^(-1/5) = m^(-1/5) 2^(-j/5) 2^(-q).
+// m^(-1/5): quadratic start (|rel. error| < 2e-3), then four Newton steps
+// y <- y (6 - m y^5) / 5 on y^-5 = m (error 2e-3 -> 1e-5 -> 4e-10 -> 5e-19).
+__device__ __forceinline__ double pow_m02(double x) {
+    if (x != x) return x;
+    if (x == 0.0) return __builtin_huge_val();
+    if (__builtin_isinf(x)) return x > 0.0 ? 0.0 : __builtin_nan("");
+    if (x < 0.0) return __builtin_nan("");
+    int e = 0;
+    double m = frexp(x, &e);
+    m = m * 2.0;
+    e = e - 1;
+    int q = e >= 0 ? e / 5 : -((-e + 4) / 5);
+    int j = e - 5 * q;
+    double tj = 1.0;
+    tj = j == 1 ? 0x1.bdb8cdadbe120p-1 : tj;   // 2^(-1/5)
+    tj = j == 2 ? 0x1.8406003b2ae5cp-1 : tj;   // 2^(-2/5)
+    tj = j == 3 ? 0x1.51cb453b9536cp-1 : tj;   // 2^(-3/5)
+    tj = j == 4 ? 0x1.2611186bae675p-1 : tj;   // 2^(-4/5)
+    double y = fma(fma(0.0517578125, m, -0.283203125), m, 1.2294921875);
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        double y2 = y * y;
+        double y5 = y2 * y2 * y;
+        y = y * (6.0 - m * y5) * 0.2;
+    }
+    return ldexp(y * tj, -q);
+}
+
+// cuda/kerr_math.cuh:50-59: Kerr-Schild radius from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0.
+__device__ __forceinline__ double ks_r(double x, double y, double z, double a2) {
+    double rho2 = fma(x, x, fma(y, y, z * z));
+    double u = rho2 - a2;
+    double s_sq = fma(u, u, 4.0 * a2 * y * y);
+    double inv_s = 1.0 / sqrt(s_sq);
+    double s = s_sq * inv_s;
+    double r2 = 0.5 * (u + s);
+    return sqrt(r2);
+}
+
+struct V6 {
+    double x, y, z, px, py, pz;
+};
+
+// Geometry of one render that the right-hand side reads.
+struct Geo {
+    double a, aa, M;
+};
+
+// cuda/kerr_math.cuh:82-156: Hamiltonian right-hand side in Cartesian
+// Kerr-Schild coordinates, dx/dl = p - 2H (l.p - pt) l and
+// dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p).
+__device__ __forceinline__ V6 rhs(const V6& s, double pt, const Geo& g) {
+    const double x = s.x, y = s.y, z = s.z;
+    const double a2 = g.aa;
+    double rho2 = fma(x, x, fma(y, y, z * z));
+    double u = rho2 - a2;
+    double s_sq = fma(u, u, 4.0 * a2 * y * y);
+    double inv_s = 1.0 / sqrt(s_sq);
+    double r2 = 0.5 * (u + s_sq * inv_s);
+    double inv_r = 1.0 / sqrt(r2);
+    double r = r2 * inv_r;
+    double dr2_dx = x * (1.0 + u * inv_s);
+    double dr2_dy = y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
+    double dr2_dz = z * (1.0 + u * inv_s);
+    double inv_2r = 0.5 * inv_r;
+    double drdx = dr2_dx * inv_2r;
+    double drdy = dr2_dy * inv_2r;
+    double drdz = dr2_dz * inv_2r;
+    double denom = r2 + a2;
+    double inv_denom = 1.0 / denom;
+    double inv_denom2 = inv_denom * inv_denom;
+    double lx = (r * x - g.a * z) * inv_denom;
+    double ly = y * inv_r;
+    double lz = (r * z + g.a * x) * inv_denom;
+    double r3 = r2 * r;
+    double denomH = fma(r2, r2, a2 * y * y);
+    double inv_denomH = 1.0 / denomH;
+    double H = g.M * r3 * inv_denomH;
+    double lp = fma(lx, s.px, fma(ly, s.py, fma(lz, s.pz, -pt)));
+    V6 d;
+    double c = 2.0 * H * lp;
+    d.x = s.px - c * lx;
+    d.y = s.py - c * ly;
+    d.z = s.pz - c * lz;
+    double dD_dx = 2.0 * r * drdx;
+    double dD_dy = 2.0 * r * drdy;
+    double dD_dz = 2.0 * r * drdz;
+    double Nx = r * x - g.a * z;
+    double Nz = r * z + g.a * x;
+    double dNx_dx = drdx * x + r;
+    double dNx_dy = drdy * x;
+    double dNx_dz = drdz * x - g.a;
+    double dNz_dx = drdx * z + g.a;
+    double dNz_dy = drdy * z;
+    double dNz_dz = drdz * z + r;
+    double dlx_dx = (dNx_dx * denom - Nx * dD_dx) * inv_denom2;
+    double dlx_dy = (dNx_dy * denom - Nx * dD_dy) * inv_denom2;
+    double dlx_dz = (dNx_dz * denom - Nx * dD_dz) * inv_denom2;
+    double dlz_dx = (dNz_dx * denom - Nz * dD_dx) * inv_denom2;
+    double dlz_dy = (dNz_dy * denom - Nz * dD_dy) * inv_denom2;
+    double dlz_dz = (dNz_dz * denom - Nz * dD_dz) * inv_denom2;
+    double inv_r2 = inv_r * inv_r;
+    double dly_dx = -y * drdx * inv_r2;
+    double dly_dy = (r - y * drdy) * inv_r2;
+    double dly_dz = -y * drdz * inv_r2;
+    double dDenH_dx = 4.0 * r3 * drdx;
+    double dDenH_dy = fma(4.0 * r3, drdy, 2.0 * a2 * y);
+    double dDenH_dz = 4.0 * r3 * drdz;
+    double inv_denomH2 = inv_denomH * inv_denomH;
+    double factor_H = g.M * inv_denomH2;
+    double dH_dx = factor_H * (3.0 * r2 * drdx * denomH - r3 * dDenH_dx);
+    double dH_dy = factor_H * (3.0 * r2 * drdy * denomH - r3 * dDenH_dy);
+    double dH_dz = factor_H * (3.0 * r2 * drdz * denomH - r3 * dDenH_dz);
+    double dlp_dx = dlx_dx * s.px + dly_dx * s.py + dlz_dx * s.pz;
+    double dlp_dy = dlx_dy * s.px + dly_dy * s.py + dlz_dy * s.pz;
+    double dlp_dz = dlx_dz * s.px + dly_dz * s.py + dlz_dz * s.pz;
+    double lp2 = lp * lp;
+    d.px = dH_dx * lp2 + c * dlp_dx;
+    d.py = dH_dy * lp2 + c * dlp_dy;
+    d.pz = dH_dz * lp2 + c * dlp_dz;
+    return d;
+}
+
+// DOPRI5 continuous extension (cuda/kerr_math.cuh:186-219; Hairer, Norsett
+// and Wanner's dense-output coefficients).
+struct Dense {
+    static constexpr double d1 = -12715105075.0 / 11282082432.0;
+    static constexpr double d3 = 87487479700.0 / 32700410799.0;
+    static constexpr double d4 = -10690763975.0 / 1880347072.0;
+    static constexpr double d5 = 701980252875.0 / 199316789632.0;
+    static constexpr double d6 = -1453857185.0 / 822651844.0;
+    static constexpr double d7 = 69997945.0 / 29380423.0;
+};
+__device__ __forceinline__ double dense(double y0, double y1, double k1, double k3, double k4, double k5,
+                                        double k6, double k7, double h, double t) {
+    double r1 = y0;
+    double r2 = y1 - y0;
+    double r3 = y0 + h * k1 - y1;
+    double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
+    double r5 = h * (Dense::d1 * k1 + Dense::d3 * k3 + Dense::d4 * k4 + Dense::d5 * k5 + Dense::d6 * k6 +
+                     Dense::d7 * k7);
+    double t2 = t * t;
+    double t3 = t2 * t;
+    double t4 = t2 * t2;
+    return r1 + t * (r2 + r3) - t2 * (r3 - r4 - r5) - t3 * (r4 + 2.0 * r5) + t4 * r5;
+}
+__device__ __forceinline__ double dense_dt(double y0, double y1, double k1, double k3, double k4, double k5,
+                                           double k6, double k7, double h, double t) {
+    double r2 = y1 - y0;
+    double r3 = y0 + h * k1 - y1;
+    double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
+    double r5 = h * (Dense::d1 * k1 + Dense::d3 * k3 + Dense::d4 * k4 + Dense::d5 * k5 + Dense::d6 * k6 +
+                     Dense::d7 * k7);
+    double t2 = t * t;
+    double t3 = t2 * t;
+    return (r2 + r3) - 2.0 * t * (r3 - r4 - r5) - 3.0 * t2 * (r4 + 2.0 * r5) + 4.0 * t3 * r5;
+}
+
+// Root of the interpolant y(t) on [0,1]: secant start, Newton steps kept
+// inside the shrinking bracket, bisection when Newton leaves it.
+__device__ __forceinline__ double disk_root(double y0, double y1, double k1, double k3, double k4, double k5,
+                                            double k6, double k7, double h) {
+    double lo = 0.0, hi = 1.0, ylo = y0;
+    double den = y0 - y1;
+    double t = den != 0.0 ? y0 / den : 0.5;
+    t = fmin(fmax(t, 0.0), 1.0);
+    for (int it = 0; it < 64; ++it) {
+        double yt = dense(y0, y1, k1, k3, k4, k5, k6, k7, h, t);
+        if (yt == 0.0) return t;
+        if ((yt >= 0.0) == (ylo >= 0.0)) {
+            lo = t;
+            ylo = yt;
+        } else {
+            hi = t;
+        }
+        double dyt = dense_dt(y0, y1, k1, k3, k4, k5, k6, k7, h, t);
+        double tn = dyt != 0.0 ? t - yt / dyt : 0.5 * (lo + hi);
+        if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        double step = tn - t;
+        t = tn;
+        if (fabs(step) <= 1e-15 || hi - lo <= 1e-15) break;
+    }
+    return t;
+}
+
+// cuda/kerr_math.cuh:172-179
+__device__ __forceinline__ double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+__device__ __forceinline__ double lerp(double a, double b, double t) { return fma(t, b - a, a); }
+
+// cuda/kerr_math.cuh:220-260: gradient noise on the integer lattice; the
+// gradient angle of a lattice point comes from pcg32 of its packed
+// coordinates (xor-ed with the seed mix, 0 = reference hash).
+__device__ __forceinline__ void lattice_gradient(int ix, int iy, uint64_t seed_mix, double& gx, double& gy) {
+    uint64_t st = ((uint64_t)(uint32_t)ix << 32) | (uint64_t)(uint32_t)iy;
+    st ^= seed_mix;
+    uint32_t hsh = pcg32(st);
+    double angle = (double)hsh * (2.0 * 3.14159265358979323846 * 0x1p-32);
+    sincos(angle, &gy, &gx);
+}
+__device__ __forceinline__ double gradient_noise(double u, double v, uint64_t seed_mix) {
+    double fu = floor(u), fv = floor(v);
+    int ix0 = (int)fu, iy0 = (int)fv;
+    double fx = u - (double)ix0;
+    double fy = v - (double)iy0;
+    double g00x, g00y, g10x, g10y, g01x, g01y, g11x, g11y;
+    lattice_gradient(ix0, iy0, seed_mix, g00x, g00y);
+    lattice_gradient(ix0 + 1, iy0, seed_mix, g10x, g10y);
+    lattice_gradient(ix0, iy0 + 1, seed_mix, g01x, g01y);
+    lattice_gradient(ix0 + 1, iy0 + 1, seed_mix, g11x, g11y);
+    double n00 = g00x * fx + g00y * fy;
+    double n10 = g10x * (fx - 1.0) + g10y * fy;
+    double n01 = g01x * fx + g01y * (fy - 1.0);
+    double n11 = g11x * (fx - 1.0) + g11y * (fy - 1.0);
+    double u_f = fade(fx);
+    double v_f = fade(fy);
+    double x1 = lerp(n00, n10, u_f);
+    double x2 = lerp(n01, n11, u_f);
+    return lerp(x1, x2, v_f);
+}
+// cuda/kerr_math.cuh:261-281
+__device__ __forceinline__ double fbm(double u, double v, int octaves, uint64_t seed_mix) {
+    if (octaves <= 0) return 0.5;
+    double sum = 0.0, amp = 1.0, freq = 1.0, amp_sum = 0.0;
+    for (int i = 0; i < octaves; ++i) {
+        sum = sum + gradient_noise(u * freq, v * freq, seed_mix) * amp;
+        amp_sum += amp;
+        amp *= 0.5;
+        freq *= 2.0;
+    }
+    double vv = amp_sum > 0.0 ? (sum / amp_sum) : 0.0;
+    return vv * 0.5 + 0.5;
+}
+
+}  // namespace kt

@@ -1,0 +1,158 @@
+"""Python host over the C ABI, mirroring the reference's ``CudaRenderer``
+(src/renderer/mod.rs:52-293): ``submit_render`` / ``ready_frame`` /
+``finish_frame``, plus stream-ordered ``trace`` and ``post_process`` calls on
+torch device tensors.  PyTorch only supplies device memory and streams.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._abi import (KtCamera, KtConfig, KtDerived, KtFrameView, KtLaunchOpts,
+                   KtTraceStats)
+from ._lib import check, lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("kerrtrace_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+@dataclass
+class FrameView:
+    """src/renderer/mod.rs:20-23: index + borrowed 0x00RRGGBB pixels."""
+    index: int
+    data: np.ndarray  # uint32 [H, W], valid until finish_frame(index)
+
+
+class HipRenderer:
+    """One renderer handle (one HIP stream, two frame buffers)."""
+
+    def __init__(self, cfg: KtConfig):
+        _torch()
+        self.cfg = cfg.copy()
+        h = C.c_void_p()
+        check(lib().kt_create(C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        d = KtDerived()
+        check(lib().kt_get_derived(self._h, C.byref(d)), self._h)
+        self.derived = d
+
+    # ---- lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def width(self):
+        return self.cfg.width
+
+    @property
+    def height(self):
+        return self.cfg.height
+
+    # ---- stream-ordered device API
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0):
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, 0)
+        check(lib().kt_set_launch_opts(self._h, C.byref(o)), self._h)
+
+    def alloc_hdr(self):
+        torch = _torch()
+        return torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
+
+    def trace(self, cam: KtCamera, hdr=None, cls=None, with_class=True, stream=None):
+        """Trace one frame on the current torch stream.  Returns (hdr, cls):
+        float32 [H, W, 4] RGBA (A = 1) and uint8 [H, W] hit classes."""
+        torch = _torch()
+        if hdr is None:
+            hdr = self.alloc_hdr()
+        if cls is None and with_class:
+            cls = torch.empty((self.height, self.width), dtype=torch.uint8, device="cuda")
+        _check_tensor(hdr, (self.height, self.width, 4), torch.float32)
+        if cls is not None:
+            _check_tensor(cls, (self.height, self.width), torch.uint8)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().kt_trace(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()),
+                             C.c_void_p(cls.data_ptr() if cls is not None else 0),
+                             C.c_void_p(s.cuda_stream)), self._h)
+        return hdr, cls
+
+    def stats(self):
+        """Counters of the last trace (synchronises its stream)."""
+        st = KtTraceStats()
+        check(lib().kt_get_trace_stats(self._h, C.byref(st)), self._h)
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def post_process(self, hdr, ldr=None, bloom=None, stream=None):
+        """Bloom + ACES + sRGB -> int32 [H, W] holding 0x00RRGGBB."""
+        torch = _torch()
+        _check_tensor(hdr, (self.height, self.width, 4), torch.float32)
+        if ldr is None:
+            ldr = torch.empty((self.height, self.width), dtype=torch.int32, device="cuda")
+        if bloom is None and self.derived.bloom_active:
+            bloom = torch.empty_like(hdr)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().kt_post_process(self._h, C.c_void_p(hdr.data_ptr()),
+                                    C.c_void_p(bloom.data_ptr() if bloom is not None else 0),
+                                    C.c_void_p(ldr.data_ptr()), C.c_void_p(s.cuda_stream)), self._h)
+        return ldr
+
+    def luts(self):
+        n = self.cfg.lut_size
+        disk = np.zeros(n, dtype=np.float64)
+        bb = np.zeros(n * 3, dtype=np.float64)
+        check(lib().kt_get_luts(self._h, disk.ctypes.data_as(C.POINTER(C.c_double)),
+                                bb.ctypes.data_as(C.POINTER(C.c_double))), self._h)
+        return disk, bb.reshape(-1, 3)
+
+    # ---- frame API (src/renderer/mod.rs:119-260)
+    def submit_render(self, cam_pos, fwd, rgt, up, fov_scale):
+        f3 = C.c_float * 3
+        ok = C.c_int32(0)
+        check(lib().kt_submit_render(self._h, f3(*cam_pos), f3(*fwd), f3(*rgt), f3(*up),
+                                     C.c_float(fov_scale), C.byref(ok)), self._h)
+        return bool(ok.value)
+
+    def submit_render_camera(self, cam: KtCamera):
+        ok = C.c_int32(0)
+        check(lib().kt_submit_render_f64(self._h, C.byref(cam), C.byref(ok)), self._h)
+        return bool(ok.value)
+
+    def ready_frame(self):
+        v = KtFrameView()
+        has = C.c_int32(0)
+        check(lib().kt_ready_frame(self._h, C.byref(v), C.byref(has)), self._h)
+        if not has.value:
+            return None
+        n = v.width * v.height
+        arr = np.ctypeslib.as_array(v.data, shape=(n,)).reshape(v.height, v.width)
+        return FrameView(index=v.index, data=arr)
+
+    def finish_frame(self, index):
+        check(lib().kt_finish_frame(self._h, int(index)), self._h)
+
+
+def _check_tensor(t, shape, dtype):
+    if tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"expected a contiguous cuda tensor {shape} {dtype}, got {tuple(t.shape)} {t.dtype}")
+
+
+def unpack_ldr(ldr):
+    """0x00RRGGBB (int32/uint32 array) -> uint8 [H, W, 3]."""
+    a = np.asarray(ldr).astype(np.uint32)
+    return np.stack([(a >> 16) & 255, (a >> 8) & 255, a & 255], axis=-1).astype(np.uint8)

@@ -1,0 +1,192 @@
+"""GPU parity of the FP64 tracer and the post-processing against the FP64
+CPU oracle, through the C ABI (libkerrtrace.so)."""
+import sys
+
+import numpy as np
+import pytest
+
+from kerrtrace_amd import config as K
+from oracle import oracle as O
+from tests.parity import compare, ldr_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _render_gpu(torch, cfg, cam):
+    from kerrtrace_amd.renderer import HipRenderer
+    with HipRenderer(cfg) as r:
+        hdr, cls = r.trace(cam)
+        torch.cuda.synchronize()
+        st = r.stats()
+        return hdr.cpu().numpy(), cls.cpu().numpy(), st
+
+
+# Tolerances of the FP64 mode: hit classes may differ only on boundary
+# pixels and on at most 0.1% of pixels; HDR relative error <= 1e-5 on
+# pixels of equal class (the HDR buffer itself is float32, 6e-8).
+FP64_MAX_MISMATCH_FRAC = 1e-3
+FP64_MAX_REL = 1e-5
+
+
+@pytest.mark.parametrize("name,size", [("cpu256_a09", None), ("hd1080_a0998", (480, 270))])
+def test_fp64_frame_matches_oracle(torch_gpu, name, size):
+    cfg, cam = K.preset(name)
+    if size is not None:
+        cfg, cam = K.scaled(cfg, cam, *size)
+    say(f"[parity] fp64 {name} {cfg.width}x{cfg.height}")
+    g_hdr, g_cls, st = _render_gpu(torch_gpu, cfg, cam)
+    o_hdr, o_cls, o_att, ost = O.render(cfg, cam)
+    m = compare(g_hdr, g_cls, o_hdr, o_cls)
+    say(f"[parity] {name}: {m}; gpu attempts {st['attempts']} oracle {ost.attempts}")
+    assert m["class_mismatch_non_boundary"] == 0
+    assert m["class_mismatch_frac"] <= FP64_MAX_MISMATCH_FRAC
+    assert m["max_rel"] <= FP64_MAX_REL
+    # identical trajectories give identical work counters (allowing for a
+    # shading-only difference that moves a transmittance cutoff)
+    assert abs(int(st["attempts"]) - int(ost.attempts)) <= 1e-4 * ost.attempts
+    assert abs(int(st["crossings"]) - int(ost.crossings)) <= 1e-4 * max(ost.crossings, 1)
+    assert np.all(g_hdr[..., 3] == 1.0)
+
+
+def test_secant_reference_root_mode(torch_gpu):
+    """The as-written crossing (clamped secant, tol floor 0) on the 256^2 scene."""
+    cfg, cam = K.preset("cpu256_a09")
+    cfg.disk_root_mode = K.KT_DISK_ROOT_SECANT_REF
+    cfg.tol_floor = 0.0
+    cfg, cam = K.scaled(cfg, cam, 128, 128)
+    say("[parity] fp64 secant-root tol_floor=0 128x128")
+    g_hdr, g_cls, st = _render_gpu(torch_gpu, cfg, cam)
+    o_hdr, o_cls, _, ost = O.render(cfg, cam)
+    m = compare(g_hdr, g_cls, o_hdr, o_cls)
+    say(f"[parity] secant: {m}")
+    assert m["class_mismatch_non_boundary"] == 0
+    assert m["max_rel"] <= FP64_MAX_REL
+
+
+def test_reference_defaults_scene(torch_gpu):
+    """config.yaml defaults (spin -0.45, M 0.5, spp 4, 100 steps) at 210x90."""
+    cfg = K.default_config()
+    cfg.width, cfg.height = 210, 90
+    tree = {"pos": (80.0, 6.0, 0.0)}
+    cam = K.camera_from_yaw_pitch(K.clamp_to_escape_radius(tree["pos"], cfg.escape_radius), -96.0, -4.0, 15.0)
+    say("[parity] fp64 reference defaults 210x90 spp4")
+    g_hdr, g_cls, st = _render_gpu(torch_gpu, cfg, cam)
+    o_hdr, o_cls, _, ost = O.render(cfg, cam)
+    m = compare(g_hdr, g_cls, o_hdr, o_cls)
+    say(f"[parity] defaults: {m}")
+    assert m["class_mismatch_frac"] <= 2e-3
+    assert m["max_rel"] <= FP64_MAX_REL
+
+
+def test_post_process_matches_oracle(torch_gpu):
+    """Bloom + ACES + sRGB of the GPU HDR frame, +-1 LSB of the FP64 oracle."""
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("cpu256_a09")
+    say("[parity] post-process cpu256_a09")
+    with HipRenderer(cfg) as r:
+        hdr, _ = r.trace(cam, with_class=False)
+        ldr = r.post_process(hdr)
+        torch.cuda.synchronize()
+        h = hdr.cpu().numpy()
+        g_ldr = ldr.cpu().numpy()
+    o_ldr, _ = O.post(cfg, h)
+    d, hist = ldr_diff(g_ldr, o_ldr)
+    say(f"[parity] post max byte diff {d} hist {hist}")
+    assert d <= 1
+
+
+def test_post_process_no_bloom_and_threshold(torch_gpu):
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipRenderer
+    rng = np.random.default_rng(3)
+    for enabled, thr, radius in [(0, 0.0, 24.0), (1, 0.0, 7.5), (1, 0.3, 36.0)]:
+        cfg, cam = K.preset("cpu256_a09")
+        cfg.width, cfg.height = 300, 77
+        cfg.bloom_enabled, cfg.bloom_threshold, cfg.bloom_radius = enabled, thr, radius
+        say(f"[parity] post synthetic enabled={enabled} thr={thr} R={radius}")
+        h = np.ones((77, 300, 4), dtype=np.float32)
+        h[..., :3] = (rng.random((77, 300, 3)) ** 3 * 4.0).astype(np.float32)
+        with HipRenderer(cfg) as r:
+            t = torch.from_numpy(h).cuda()
+            g_ldr = r.post_process(t).cpu().numpy()
+        o_ldr, _ = O.post(cfg, h)
+        d, hist = ldr_diff(g_ldr, o_ldr)
+        say(f"[parity]   max diff {d} hist {hist}")
+        assert d <= 1
+
+
+def test_frame_api_sequencing(torch_gpu):
+    """submit/ready/finish semantics of src/renderer/mod.rs:119-260."""
+    from kerrtrace_amd._lib import KerrTraceError
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("cpu256_a09")
+    cfg, cam = K.scaled(cfg, cam, 64, 48)
+    say("[parity] frame api 64x48")
+    with HipRenderer(cfg) as r:
+        assert r.submit_render_camera(cam)
+        assert r.submit_render_camera(cam)
+        assert not r.submit_render_camera(cam)  # both frames in flight
+        torch_gpu.cuda.synchronize()
+        import time
+        f = None
+        for _ in range(2000):
+            f = r.ready_frame()
+            if f is not None:
+                break
+            time.sleep(0.005)
+        assert f is not None and f.index == 0
+        first = f.data.copy()
+        with pytest.raises(KerrTraceError) as ei:
+            r.finish_frame(1)  # out of order
+        assert ei.value.code == -5
+        r.finish_frame(0)
+        f2 = None
+        for _ in range(2000):
+            f2 = r.ready_frame()
+            if f2 is not None:
+                break
+            time.sleep(0.005)
+        assert f2 is not None and f2.index == 1
+        assert np.array_equal(first, f2.data)  # same camera, deterministic
+        r.finish_frame(1)
+        with pytest.raises(KerrTraceError):
+            r.finish_frame(0)  # empty queue
+        # the frame through the f32 entry point equals the trace + post path
+        pos = [float(v) for v in cam.pos]
+        ok = r.submit_render(pos, list(cam.fwd), list(cam.rgt), list(cam.up), float(cam.fov_scale))
+        assert ok
+
+
+def test_lut_overflow_rejects_frame(torch_gpu):
+    """A temperature above lut_max_temp fails the frame (kernel.cu:3-10, mod.rs:236-241)."""
+    from kerrtrace_amd._lib import KerrTraceError
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("cpu256_a09")
+    cfg, cam = K.scaled(cfg, cam, 64, 64)
+    cfg.lut_max_temp = 2000.0
+    say("[parity] lut overflow 64x64")
+    with HipRenderer(cfg) as r:
+        assert r.submit_render_camera(cam)
+        torch_gpu.cuda.synchronize()
+        import time
+        err = None
+        for _ in range(2000):
+            try:
+                f = r.ready_frame()
+            except KerrTraceError as e:
+                err = e
+                break
+            if f is not None:
+                break
+            time.sleep(0.005)
+        assert err is not None and err.code == -4
+        hdr, cls = r.trace(cam)
+        st = r.stats()
+        assert st["lut_overflow"] == 1 and st["overflow_temperature"] > 2000.0
+        assert (cls.cpu().numpy() & 16).any()
