@@ -69,6 +69,15 @@ enum {
                                     dense-output interpolant, in FP64 */
 };
 
+/* Right-hand side of the ray equations (cuda/kerr_math.cuh:82-156). */
+enum {
+    KT_RHS_REFERENCE = 0,  /* as written: dr^2/dy = y (1 + (u + 4a^2) / (2s)),
+                              which halves the u/s term of the exact
+                              derivative (kerr_math.cuh:97) */
+    KT_RHS_HAMILTONIAN = 1 /* exact gradient of the Kerr-Schild Hamiltonian:
+                              dr^2/dy = y (1 + (u + 2a^2) / s) */
+};
+
 /* Per-pixel hit-class byte written by kt_trace (new; the reference only
  * writes colour).  Bits 0-1: terminal reason; higher bits: flags. */
 enum {
@@ -134,6 +143,8 @@ typedef struct kt_config {
     double mdot_scale;                 /* ext: temperature_scale *= mdot_scale^(1/4); 1 == reference */
     int32_t disk_root_mode;            /* ext: KT_DISK_ROOT_* */
     int32_t precision;                 /* ext: KT_PRECISION_* */
+    int32_t rhs_mode;                  /* ext: KT_RHS_* */
+    int32_t reserved2;
 } kt_config;
 
 /* Pinhole camera in FP64.  The reference passes 12 f32 scalars plus

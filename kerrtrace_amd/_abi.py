@@ -19,6 +19,8 @@ KT_PRECISION_FP64 = 0
 KT_PRECISION_MIXED = 1
 KT_DISK_ROOT_SECANT_REF = 0
 KT_DISK_ROOT_TRUE = 1
+KT_RHS_REFERENCE = 0
+KT_RHS_HAMILTONIAN = 1
 
 KT_END_OTHER = 0
 KT_END_HORIZON = 1
@@ -70,6 +72,8 @@ class KtConfig(C.Structure):
         ("mdot_scale", C.c_double),
         ("disk_root_mode", C.c_int32),
         ("precision", C.c_int32),
+        ("rhs_mode", C.c_int32),
+        ("reserved2", C.c_int32),
     ]
 
     def copy(self):

@@ -16,7 +16,8 @@ import math
 import yaml
 
 from ._abi import (KT_DISK_ROOT_SECANT_REF, KT_DISK_ROOT_TRUE,
-                   KT_PRECISION_FP64, KT_PRECISION_MIXED, KtCamera, KtConfig)
+                   KT_PRECISION_FP64, KT_PRECISION_MIXED, KT_RHS_HAMILTONIAN,
+                   KT_RHS_REFERENCE, KtCamera, KtConfig)
 
 U32_MAX = 0xFFFFFFFF
 
@@ -100,6 +101,7 @@ def default_config():
     c.mdot_scale = 1.0
     c.disk_root_mode = KT_DISK_ROOT_SECANT_REF
     c.precision = KT_PRECISION_FP64
+    c.rhs_mode = KT_RHS_REFERENCE
     return c
 
 
@@ -224,6 +226,10 @@ def config_from_dict(tree):
     if root not in ("secant_ref", "true_root"):
         raise ConfigError(f"kerrtrace_amd.disk_root must be secant_ref or true_root, got {root}")
     c.disk_root_mode = KT_DISK_ROOT_TRUE if root == "true_root" else KT_DISK_ROOT_SECANT_REF
+    rhs = str(ext.get("rhs", "reference")).lower()
+    if rhs not in ("reference", "hamiltonian"):
+        raise ConfigError(f"kerrtrace_amd.rhs must be reference or hamiltonian, got {rhs}")
+    c.rhs_mode = KT_RHS_HAMILTONIAN if rhs == "hamiltonian" else KT_RHS_REFERENCE
     c.noise_seed = int(ext.get("noise_seed", 0))
     c.bloom_threshold = parse_float(ext.get("bloom_threshold", 0.0), "bloom_threshold")
     c.tol_floor = parse_float(ext.get("tol_floor", 0.0), "tol_floor")
@@ -252,9 +258,9 @@ def load_config(path):
 # Presets: the measured and checked workloads, named by what they are.
 # Knobs a workload does not state take the reference defaults, except:
 # spp = 1, tol_floor = 1 (tolerance never tighter than stated), disk root
-# solved on the interpolant, escape radius = observer distance (the camera
-# sits on the escape sphere as src/app.rs:126-139 enforces), bloom
-# radius = 3 sigma.
+# solved on the interpolant, exact Hamiltonian right-hand side, escape
+# radius = observer distance (the camera sits on the escape sphere as
+# src/app.rs:126-139 enforces), bloom radius = 3 sigma.
 
 def _observer_preset(width, height, spin, incl, r_obs, fov, tol, r_out,
                      octaves, seed, sigma, threshold=1.0, peak_t=1e4,
@@ -274,6 +280,7 @@ def _observer_preset(width, height, spin, incl, r_obs, fov, tol, r_out,
     c.bloom_threshold = threshold
     c.tol_floor = 1.0
     c.disk_root_mode = KT_DISK_ROOT_TRUE
+    c.rhs_mode = KT_RHS_HAMILTONIAN
     c.precision = precision
     c.mdot_scale = mdot
     return c, observer_camera(r_obs, incl, fov)

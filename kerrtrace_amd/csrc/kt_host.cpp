@@ -163,6 +163,7 @@ int derive(const kt_config& c, Derived& d, std::string& why) {
     if (!(std::isfinite(c.mdot_scale) && c.mdot_scale >= 0.0)) { why = "mdot_scale must be finite and non-negative"; return KT_ERR_INVALID_CONFIG; }
     if (c.precision != KT_PRECISION_FP64 && c.precision != KT_PRECISION_MIXED) { why = "unknown precision mode"; return KT_ERR_INVALID_CONFIG; }
     if (c.disk_root_mode != KT_DISK_ROOT_SECANT_REF && c.disk_root_mode != KT_DISK_ROOT_TRUE) { why = "unknown disk root mode"; return KT_ERR_INVALID_CONFIG; }
+    if (c.rhs_mode != KT_RHS_REFERENCE && c.rhs_mode != KT_RHS_HAMILTONIAN) { why = "unknown rhs mode"; return KT_ERR_INVALID_CONFIG; }
     if ((uint64_t)c.width * c.height > (1ull << 31)) { why = "frame larger than 2^31 pixels"; return KT_ERR_CAPACITY; }
     const double M = c.mass;
     const double inv_m = 1.0 / M;
@@ -358,6 +359,7 @@ void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, con
     bool noise_on = c.noise_enabled != 0 && c.noise_detail > 0 && c.noise_strength != 0.0;
     P.noise_detail = noise_on ? (int)c.noise_detail : 0;
     P.disk_root_mode = c.disk_root_mode;
+    P.rhs_exact = c.rhs_mode == KT_RHS_HAMILTONIAN ? 1 : 0;
     P.tile = tile;
     P.tiles_x = ((int)c.width + tile - 1) / tile;
     int tiles_y = ((int)c.height + tile - 1) / tile;
@@ -459,6 +461,7 @@ kt_status kt_config_defaults(kt_config* c) {
     c->mdot_scale = 1.0;
     c->disk_root_mode = KT_DISK_ROOT_SECANT_REF;
     c->precision = KT_PRECISION_FP64;
+    c->rhs_mode = KT_RHS_REFERENCE;
     return KT_OK;
 }
 

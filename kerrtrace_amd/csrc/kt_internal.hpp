@@ -35,6 +35,8 @@ struct RenderParams {
     uint8_t* cls;             // may be null
     uint32_t* error_flag;     // [0] set once, [1] float bits of the first overflowing T
     int32_t W, H, spp, max_steps, max_attempts, lut_n, noise_detail, disk_root_mode;
+    int32_t rhs_exact;        // 1: exact Hamiltonian dr^2/dy
+    int32_t reserved;
     uint64_t ray_begin;       // first global work item of this render (batched launches)
     uint64_t ray_count;       // work items of this render: W * H
     int32_t tile;             // work-queue tile edge (pixels)

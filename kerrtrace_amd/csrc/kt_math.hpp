@@ -72,6 +72,7 @@ struct V6 {
 // Geometry of one render that the right-hand side reads.
 struct Geo {
     double a, aa, M;
+    bool exact;  // exact Hamiltonian dr^2/dy instead of the as-written one
 };
 
 // cuda/kerr_math.cuh:82-156: Hamiltonian right-hand side in Cartesian
@@ -88,7 +89,9 @@ __device__ __forceinline__ V6 rhs(const V6& s, double pt, const Geo& g) {
     double inv_r = 1.0 / sqrt(r2);
     double r = r2 * inv_r;
     double dr2_dx = x * (1.0 + u * inv_s);
-    double dr2_dy = y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
+    // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
+    // exact derivative of r^2 is y (1 + (u + 2a^2)/s)
+    double dr2_dy = g.exact ? y * (1.0 + (u + 2.0 * a2) * inv_s) : y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
     double dr2_dz = z * (1.0 + u * inv_s);
     double inv_2r = 0.5 * inv_r;
     double drdx = dr2_dx * inv_2r;

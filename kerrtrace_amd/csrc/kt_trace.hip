@@ -187,7 +187,7 @@ __device__ __noinline__ bool shade_crossing(const RenderParams& P, double x_hit,
 
 // Sky grid intensity along the escape direction (cuda/kerr_math.cuh:157-171).
 __device__ __noinline__ double sky(const RenderParams& P, const V6& s, double pt, double Tr, bool& grid) {
-    Geo g{P.a, P.aa, P.M};
+    Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
     V6 ke = rhs(s, pt, g);
     double q = ke.x * ke.x + ke.y * ke.y + ke.z * ke.z;
     double inv = 1.0 / sqrt(q);
@@ -263,7 +263,7 @@ struct Hot {
 
 __device__ __forceinline__ Hot load_hot(const RenderParams& P) {
     Hot o;
-    o.g = Geo{P.a, P.aa, P.M};
+    o.g = Geo{P.a, P.aa, P.M, P.rhs_exact != 0};
     o.tol = P.tol;
     o.tol_floor = P.tol_floor;
     o.cutoff = P.cutoff;

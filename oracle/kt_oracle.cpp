@@ -154,6 +154,7 @@ struct Params {
     double wl_step, lut_max_temp;
     int lut_n;
     int disk_root_mode;
+    bool rhs_exact;
     std::vector<S> disk_lut;
     std::vector<double> bb_lut;
     double bloom_sigma;
@@ -244,6 +245,7 @@ static int derive(const kt_config& c, S spin_s, S mdot_s, Params<S>& p, const ch
     p.wl_step = c.wavelength_step;
     p.lut_max_temp = c.lut_max_temp;
     p.disk_root_mode = c.disk_root_mode;
+    p.rhs_exact = c.rhs_mode == KT_RHS_HAMILTONIAN;
     int R = 0;
     if (c.bloom_radius > 0.0) R = (int)std::ceil(c.bloom_radius);
     p.bloom_R = R;
@@ -308,7 +310,9 @@ State<S> get_derivs(const State<S>& s, S pt, const Params<S>& p) {
     S inv_r = 1.0 / sqrt_(r2);
     S r = r2 * inv_r;
     S dr2_dx = x * (1.0 + u * inv_s);
-    S dr2_dy = y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
+    // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
+    // Hamiltonian mode uses the exact derivative y (1 + (u + 2a^2)/s)
+    S dr2_dy = p.rhs_exact ? y * (1.0 + (u + 2.0 * a2) * inv_s) : y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
     S dr2_dz = z * (1.0 + u * inv_s);
     S inv_2r = 0.5 * inv_r;
     S drdx = dr2_dx * inv_2r;
@@ -542,6 +546,71 @@ double disk_root_true(const State<S>& k1, const State<S>& k3, const State<S>& k4
     return t;
 }
 
+// DP5(4) tableau (kerr_trace.cuh:52-59; docs/dormand_prince_extended_tableau.md)
+struct Tab {
+    static constexpr double a21 = 1.0 / 5.0;
+    static constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+    static constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+    static constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0,
+                            a54 = -212.0 / 729.0;
+    static constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0, a64 = 49.0 / 176.0,
+                            a65 = -5103.0 / 18656.0;
+    static constexpr double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0,
+                            b6 = 11.0 / 84.0;
+    static constexpr double dc1 = 35.0 / 384.0 - 5179.0 / 57600.0, dc3 = 500.0 / 1113.0 - 7571.0 / 16695.0,
+                            dc4 = 125.0 / 192.0 - 393.0 / 640.0, dc5 = -2187.0 / 6784.0 + 92097.0 / 339200.0,
+                            dc6 = 11.0 / 84.0 - 187.0 / 2100.0, dc7 = -1.0 / 40.0;
+};
+
+// One DP5(4) attempt from s with step hs and stage-1 derivative k1
+// (kerr_trace.cuh:85-116): stages k2..k7, the 5th-order solution ns (which
+// is also the 7th stage input, a7j = b_j) and the max-norm error of the
+// embedded 4th-order estimate relative to tol * max(|s_i|, 1).
+template <class S>
+struct Attempt {
+    State<S> k1, k2, k3, k4, k5, k6, k7, ns;
+    S error;
+};
+template <class S>
+void dp_attempt(const State<S>& s, S pt, S hs, double tol, const State<S>& k1, const Params<S>& p, Attempt<S>& A) {
+    typedef Tab T;
+    A.k1 = k1;
+    State<S>& k2 = A.k2; State<S>& k3 = A.k3; State<S>& k4 = A.k4; State<S>& k5 = A.k5;
+    State<S>& k6 = A.k6; State<S>& k7 = A.k7;
+#define KTO_STAGE(E) State<S>{s.x + hs * (E(x)), s.y + hs * (E(y)), s.z + hs * (E(z)), s.px + hs * (E(px)), s.py + hs * (E(py)), s.pz + hs * (E(pz))}
+#define E2(c) (T::a21 * k1.c)
+#define E3(c) (T::a31 * k1.c + T::a32 * k2.c)
+#define E4(c) (T::a41 * k1.c + T::a42 * k2.c + T::a43 * k3.c)
+#define E5(c) (T::a51 * k1.c + T::a52 * k2.c + T::a53 * k3.c + T::a54 * k4.c)
+#define E6(c) (T::a61 * k1.c + T::a62 * k2.c + T::a63 * k3.c + T::a64 * k4.c + T::a65 * k5.c)
+#define E7(c) (T::b1 * k1.c + T::b3 * k3.c + T::b4 * k4.c + T::b5 * k5.c + T::b6 * k6.c)
+    k2 = get_derivs(KTO_STAGE(E2), pt, p);
+    k3 = get_derivs(KTO_STAGE(E3), pt, p);
+    k4 = get_derivs(KTO_STAGE(E4), pt, p);
+    k5 = get_derivs(KTO_STAGE(E5), pt, p);
+    k6 = get_derivs(KTO_STAGE(E6), pt, p);
+    A.ns = KTO_STAGE(E7);
+    k7 = get_derivs(A.ns, pt, p);
+#undef KTO_STAGE
+#undef E2
+#undef E3
+#undef E4
+#undef E5
+#undef E6
+#undef E7
+#define KTO_ERR(c) (fabs_(hs * (T::dc1 * k1.c + T::dc3 * k3.c + T::dc4 * k4.c + T::dc5 * k5.c + T::dc6 * k6.c + T::dc7 * k7.c)) / (tol * fmax_(fabs_(s.c), S(1.0))))
+    S ex = KTO_ERR(x), ey = KTO_ERR(y), ez = KTO_ERR(z);
+    S epx = KTO_ERR(px), epy = KTO_ERR(py), epz = KTO_ERR(pz);
+#undef KTO_ERR
+    A.error = fmax_(fmax_(fmax_(ex, ey), fmax_(ez, epx)), fmax_(epy, epz));
+}
+
+// Per committed step: h used, accepted flag, state after the step.
+struct StepLog {
+    std::vector<double> rows;  // 8 values per step
+    size_t max_steps = 0;
+};
+
 template <class S>
 struct RayOut {
     S col[3];
@@ -552,7 +621,8 @@ struct RayOut {
 
 // cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
 template <class S>
-void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st) {
+void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
+               StepLog* log = nullptr) {
     // normalize (kerr_math.cuh:21-25)
     S dd = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
     S inv_n = 1.0 / sqrt_(dd);
@@ -591,17 +661,6 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
     bool overflow = false;
     double overflow_T = 0.0;
     S h(p.h0);
-    // DP5(4) tableau (kerr_trace.cuh:52-59; docs/dormand_prince_extended_tableau.md)
-    const double a21 = 1.0 / 5.0;
-    const double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
-    const double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
-    const double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0, a54 = -212.0 / 729.0;
-    const double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0, a64 = 49.0 / 176.0,
-                 a65 = -5103.0 / 18656.0;
-    const double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0, b6 = 11.0 / 84.0;
-    const double dc1 = 35.0 / 384.0 - 5179.0 / 57600.0, dc3 = 500.0 / 1113.0 - 7571.0 / 16695.0,
-                 dc4 = 125.0 / 192.0 - 393.0 / 640.0, dc5 = -2187.0 / 6784.0 + 92097.0 / 339200.0,
-                 dc6 = 11.0 / 84.0 - 187.0 / 2100.0, dc7 = -1.0 / 40.0;
     State<S> fsal_k1;
     bool have_fsal = false;
     for (int i = 0; i < p.max_steps && transmittance > p.cutoff; ++i) {
@@ -622,35 +681,11 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
         while (!accepted && attempts < p.max_attempts) {
             S hs = h;
             if (!k1_ready) { k1 = get_derivs(s, pt, p); k1_ready = true; st.rhs++; }
-#define KTO_STAGE(F, E) State<S>{s.x + hs * (E(x)), s.y + hs * (E(y)), s.z + hs * (E(z)), s.px + hs * (E(px)), s.py + hs * (E(py)), s.pz + hs * (E(pz))}
-#define E2(c) (a21 * k1.c)
-#define E3(c) (a31 * k1.c + a32 * k2.c)
-#define E4(c) (a41 * k1.c + a42 * k2.c + a43 * k3.c)
-#define E5(c) (a51 * k1.c + a52 * k2.c + a53 * k3.c + a54 * k4.c)
-#define E6(c) (a61 * k1.c + a62 * k2.c + a63 * k3.c + a64 * k4.c + a65 * k5.c)
-#define E7(c) (b1 * k1.c + b3 * k3.c + b4 * k4.c + b5 * k5.c + b6 * k6.c)
-            k2 = get_derivs(KTO_STAGE(k2, E2), pt, p);
-            k3 = get_derivs(KTO_STAGE(k3, E3), pt, p);
-            k4 = get_derivs(KTO_STAGE(k4, E4), pt, p);
-            k5 = get_derivs(KTO_STAGE(k5, E5), pt, p);
-            k6 = get_derivs(KTO_STAGE(k6, E6), pt, p);
-            // the 7th stage input equals the 5th-order solution (a7j = b_j)
-            ns = KTO_STAGE(ns, E7);
-            k7 = get_derivs(ns, pt, p);
+            Attempt<S> A;
+            dp_attempt(s, pt, hs, tol, k1, p, A);
+            k2 = A.k2; k3 = A.k3; k4 = A.k4; k5 = A.k5; k6 = A.k6; k7 = A.k7; ns = A.ns;
+            S error = A.error;
             st.rhs += 6;
-#undef KTO_STAGE
-#undef E2
-#undef E3
-#undef E4
-#undef E5
-#undef E6
-#undef E7
-            // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
-#define KTO_ERR(c) (fabs_(h * (dc1 * k1.c + dc3 * k3.c + dc4 * k4.c + dc5 * k5.c + dc6 * k6.c + dc7 * k7.c)) / (tol * fmax_(fabs_(s.c), S(1.0))))
-            S ex = KTO_ERR(x), ey = KTO_ERR(y), ez = KTO_ERR(z);
-            S epx = KTO_ERR(px), epy = KTO_ERR(py), epz = KTO_ERR(pz);
-#undef KTO_ERR
-            S error = fmax_(fmax_(fmax_(ex, ey), fmax_(ez, epx)), fmax_(epy, epz));
             st.attempts++;
             if (error <= 1.0) accepted = true;
             // controller without clamps (kerr_trace.cuh:119); the step
@@ -663,6 +698,10 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
         if (accepted) st.accepted++;
         State<S> prev = s;
         s = ns;  // committed even when rejected max_attempts times (Q3)
+        if (log && log->rows.size() / 8 < log->max_steps) {
+            double row[8] = {val(h_used), accepted ? 1.0 : 0.0, val(s.x), val(s.y), val(s.z), val(s.px), val(s.py), val(s.pz)};
+            log->rows.insert(log->rows.end(), row, row + 8);
+        }
         if (accepted) { fsal_k1 = k7; have_fsal = true; } else { have_fsal = false; }
         S r_now = ks_r(s.x, s.y, s.z, p.aa);
         if (r_now < p.rh + p.eps) { cls = (cls & ~3) | KT_END_HORIZON; break; }
@@ -987,6 +1026,49 @@ int kto_render_rows(const kt_config* cfg, const kt_camera* cam, int row_begin, i
         stats->threads = nthreads;
     }
     return KT_OK;
+}
+
+// One DP5(4) attempt on the Kerr right-hand side (for checks of the
+// stepper): out = [k1..k7 (42), ns (6), error (1)].
+int kto_dp_attempt(const kt_config* cfg, const double s6[6], double pt, double h, double tol, double* out) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    State<double> s{s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]};
+    State<double> k1 = get_derivs(s, pt, p);
+    Attempt<double> A;
+    dp_attempt(s, pt, h, tol, k1, p, A);
+    const State<double>* ks[8] = {&A.k1, &A.k2, &A.k3, &A.k4, &A.k5, &A.k6, &A.k7, &A.ns};
+    for (int i = 0; i < 8; ++i) {
+        const State<double>& v = *ks[i];
+        double* o = out + i * 6;
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.px; o[4] = v.py; o[5] = v.pz;
+    }
+    out[48] = A.error;
+    return KT_OK;
+}
+
+// Traces one ray from cam along dir.  col: rgb (before exposure scaling);
+// returns the class byte (>= 0) or an error (< 0).  log (optional): 8
+// doubles per committed step (h, accepted, x, y, z, px, py, pz), at most
+// max_log steps; *n_log receives the number written.
+int kto_trace_dir(const kt_config* cfg, const double cam[3], const double dir[3], double col[3], uint32_t stats[5],
+                  double* log, int max_log, int* n_log) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    RayOut<double> ro;
+    RayStats st;
+    StepLog L;
+    L.max_steps = log ? (size_t)std::max(max_log, 0) : 0;
+    trace_ray(cam, dir, p, ro, st, log ? &L : nullptr);
+    for (int c = 0; c < 3; ++c) col[c] = ro.col[c];
+    if (stats) { stats[0] = st.attempts; stats[1] = st.rhs; stats[2] = st.accepted; stats[3] = st.crossings; stats[4] = st.steps; }
+    if (log) {
+        std::memcpy(log, L.rows.data(), L.rows.size() * sizeof(double));
+        if (n_log) *n_log = (int)(L.rows.size() / 8);
+    }
+    return ro.cls;
 }
 
 // Bloom + ACES + sRGB + packing (cuda/kernel.cu:48-84, 144-251) on a float4

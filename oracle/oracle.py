@@ -57,8 +57,11 @@ def lib():
                                       pd, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(KtoStats)]
         L.kto_post.argtypes = [C.POINTER(KtConfig), C.POINTER(C.c_float), C.POINTER(C.c_uint32), pd]
         L.kto_flop_counts.argtypes = [C.POINTER(KtConfig), pd, d, C.POINTER(C.c_uint64)]
+        L.kto_dp_attempt.argtypes = [C.POINTER(KtConfig), pd, d, d, d, pd]
+        L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
+                                    C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
-                     "kto_flop_counts"):
+                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -182,3 +185,27 @@ def flop_counts(cfg, state6, pt):
     out = (C.c_uint64 * 4)()
     _check(lib().kto_flop_counts(C.byref(cfg), _dptr(s), pt, out))
     return int(out[0]), int(out[1]), int(out[2])
+
+
+def dp_attempt(cfg, state6, pt, h, tol):
+    """One DP5(4) attempt: (k[7,6], next_state[6], error)."""
+    s = np.ascontiguousarray(state6, dtype=np.float64)
+    out = np.zeros(49, dtype=np.float64)
+    _check(lib().kto_dp_attempt(C.byref(cfg), _dptr(s), pt, h, tol, _dptr(out)))
+    return out[:42].reshape(7, 6), out[42:48].copy(), float(out[48])
+
+
+def trace_dir(cfg, cam_pos, direction, max_log=0):
+    """One ray: (rgb before exposure, class byte, stats dict, step log [n, 8])."""
+    cam = np.ascontiguousarray(cam_pos, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    col = np.zeros(3, dtype=np.float64)
+    st = (C.c_uint32 * 5)()
+    log = np.zeros((max(max_log, 1), 8), dtype=np.float64)
+    n = C.c_int(0)
+    rc = lib().kto_trace_dir(C.byref(cfg), _dptr(cam), _dptr(d), _dptr(col), st,
+                             _dptr(log) if max_log > 0 else None, max_log, C.byref(n))
+    if rc < 0:
+        _check(rc)
+    stats = dict(zip(("attempts", "rhs", "accepted", "crossings", "steps"), list(st)))
+    return col, rc, stats, log[:n.value].copy()

@@ -1,0 +1,539 @@
+"""Pins of the FP64 oracle against what the mathematics fixes.
+
+The reference ships no tests or golden vectors for this path (its only
+artefact, picture/diagram.png, is a fast-math FP32 frame of unknown commit),
+so every oracle routine is checked here against a closed form, an invariant,
+a library routine (scipy's Dormand-Prince tableau, mpmath at 40 digits) or
+brute force -- chosen so a dropped term, a wrong sign or index, or a
+transposed operand fails at least one check.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from kerrtrace_amd import config as K
+from oracle import oracle as O
+
+mp.mp.dps = 40
+
+
+# ------------------------------------------------------------ PCG hashing
+def _pcg32_bigint(state):
+    """PCG32 XSH-RR output (O'Neill 2014) of the LCG-advanced state, with
+    Python integers (multiplier 6364136223846793005, increment
+    1442695040888963407: the PCG reference defaults)."""
+    m64 = (1 << 64) - 1
+    x = (state * 6364136223846793005 + 1442695040888963407) & m64
+    xs = (((x >> 18) ^ x) >> 27) & 0xFFFFFFFF
+    rot = x >> 59
+    return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+
+def test_pcg32_matches_bigint_xsh_rr():
+    rng = np.random.default_rng(0)
+    states = [0, 1, 2, (1 << 64) - 1, 0xda3e39cb94b95bdb, (123 << 32) | 7]
+    states += [int(v) for v in rng.integers(0, 2**63, size=200, dtype=np.int64)]
+    for s in states:
+        assert O.pcg32(s) == _pcg32_bigint(s)
+        assert O.rand01(s) == _pcg32_bigint(s) * 2.0**-32
+
+
+# -------------------------------------------------------------- ISCO, r_h
+def _circular_energy(r, a):
+    """Specific energy of a prograde equatorial circular orbit (Bardeen,
+    Press & Teukolsky 1972, eq. 2.12), M = 1."""
+    return (1 - 2 / r + a / r**1.5) / mp.sqrt(1 - 3 / r + 2 * a / r**1.5)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.3, 0.7, 0.9, 0.998])
+def test_isco_is_energy_minimum(a):
+    r = O.isco(a, True, 1.0)
+    dE = mp.diff(lambda rr: _circular_energy(rr, mp.mpf(a)), mp.mpf(r))
+    assert abs(dE) < 1e-13
+    # and it is a minimum, not a maximum
+    assert mp.diff(lambda rr: _circular_energy(rr, mp.mpf(a)), mp.mpf(r), 2) > 0
+
+
+def test_isco_known_values():
+    assert O.isco(0.0) == pytest.approx(6.0, abs=1e-14)
+    assert O.isco(1.0) == pytest.approx(1.0, abs=1e-12)
+    assert O.isco(1.0, prograde=False) == pytest.approx(9.0, abs=1e-12)
+    assert O.isco(0.9) == pytest.approx(2.3208830417618866, rel=1e-12)
+    assert round(O.isco(0.998), 4) == 1.2370
+    # odd-symmetric cbrt: a < 0 gives the same (prograde-formula) radius (utils.rs:11-20, 77)
+    assert O.isco(-0.45) == pytest.approx(O.isco(0.45), rel=1e-15)
+    assert O.isco(0.5, True, 2.0) == pytest.approx(2.0 * O.isco(0.5), rel=1e-15)
+
+
+@pytest.mark.parametrize("a,M", [(0.0, 1.0), (0.9, 1.0), (-0.45, 0.5), (0.998, 1.0)])
+def test_horizon_radius_zero_of_delta(a, M):
+    cfg = K.default_config()
+    cfg.spin, cfg.mass = a, M
+    cfg.disk_outer_radius = 40.0
+    d, _, _ = O.derive(cfg)
+    rh = d.rh
+    assert abs(rh * rh - 2 * M * rh + a * a) < 1e-13
+    assert rh >= M
+
+
+# --------------------------------------------------- controller pow(-1/5)
+def test_pow_m02_special_values():
+    assert O.pow_m02(0.0) == math.inf
+    assert O.pow_m02(math.inf) == 0.0
+    assert math.isnan(O.pow_m02(math.nan))
+    assert math.isnan(O.pow_m02(-1.0))
+    assert O.pow_m02(1.0) == 1.0
+    for k in range(-40, 41):
+        assert O.pow_m02(2.0 ** (5 * k)) == 2.0 ** (-k)
+
+
+def test_pow_m02_accuracy_against_mpmath():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 2000), rng.uniform(0.5, 2.0, 2000),
+                         [5e-324, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    worst = 0.0
+    for x in xs:
+        got = O.pow_m02(float(x))
+        ref = mp.mpf(float(x)) ** (mp.mpf(-1) / 5)
+        ulp = math.ulp(float(ref))
+        worst = max(worst, float(abs(mp.mpf(got) - ref)) / ulp)
+    assert worst <= 3.0  # a few ulp: the controller needs determinism, not correct rounding
+
+
+# ------------------------------------------------- Kerr-Schild geometry
+def _ks_r_mp(x, y, z, a):
+    x, y, z, a = (mp.mpf(v) for v in (x, y, z, a))
+    u = x * x + y * y + z * z - a * a
+    return mp.sqrt((u + mp.sqrt(u * u + 4 * a * a * y * y)) / 2)
+
+
+def test_ks_radius_root_and_limits():
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        x, y, z = rng.uniform(-30, 30, 3)
+        a = rng.uniform(-1, 1)
+        r = O.ks_r(x, y, z, a * a)
+        rho2 = x * x + y * y + z * z
+        # r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0
+        res = r**4 - (rho2 - a * a) * r**2 - a * a * y * y
+        assert abs(res) <= 1e-12 * max(r**4, 1.0)
+        assert r == pytest.approx(float(_ks_r_mp(x, y, z, a)), rel=1e-13)
+    # a = 0: Euclidean radius
+    assert O.ks_r(3.0, 4.0, 12.0, 0.0) == 13.0
+    # on the spin axis r = |y|
+    assert O.ks_r(0.0, 5.0, 0.0, 0.81) == pytest.approx(5.0, rel=1e-15)
+
+
+def _hamiltonian_mp(s, pt, a, M):
+    """H = 1/2 (p.p - pt^2) - H_ks (l.p - pt)^2: the Kerr-Schild form
+    g^{mu nu} = eta^{mu nu} - 2 H l^mu l^nu of the null geodesic
+    Hamiltonian, spin along +y."""
+    x, y, z, px, py, pz = s
+    a = mp.mpf(a)
+    r = _ks_r_mp(x, y, z, a)
+    den = r * r + a * a
+    lx, ly, lz = (r * x - a * z) / den, y / r, (r * z + a * x) / den
+    Hks = M * r**3 / (r**4 + a * a * y * y)
+    lp = lx * px + ly * py + lz * pz - pt
+    return (px * px + py * py + pz * pz - pt * pt) / 2 - Hks * lp * lp
+
+
+@pytest.mark.parametrize("a", [0.0, 0.6, -0.9, 0.998])
+def test_rhs_is_hamiltonian_gradient(a):
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.spin = a
+    rng = np.random.default_rng(3)
+    for _ in range(12):
+        pos = rng.uniform(-12, 12, 3)
+        if np.linalg.norm(pos) < 3:
+            pos *= 3 / np.linalg.norm(pos)
+        mom = rng.normal(size=3)
+        pt = -abs(rng.normal()) - 0.3
+        s = np.concatenate([pos, mom])
+        got = O.get_derivs(cfg, s, pt)
+        sm = [mp.mpf(float(v)) for v in s]
+        ptm = mp.mpf(float(pt))
+        for i in range(3):
+            # dx/dl = dH/dp,  dp/dl = -dH/dx
+            dHdp = mp.diff(lambda v: _hamiltonian_mp(sm[:3 + i] + [v] + sm[4 + i:], ptm, a, 1), sm[3 + i])
+            dHdx = mp.diff(lambda v: _hamiltonian_mp(sm[:i] + [v] + sm[i + 1:], ptm, a, 1), sm[i])
+            assert got[i] == pytest.approx(float(dHdp), rel=1e-11, abs=1e-13)
+            assert got[3 + i] == pytest.approx(float(-dHdx), rel=1e-10, abs=1e-13)
+
+
+def test_initial_momentum_is_null():
+    cfg, cam = K.preset("hd1080_a0998")
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        d = np.array(cam.fwd) + rng.uniform(-0.3, 0.3) * np.array(cam.rgt) + rng.uniform(-0.3, 0.3) * np.array(cam.up)
+        s, pt, kt = O.init_ray(cfg, list(cam.pos), d)
+        Hv = _hamiltonian_mp([mp.mpf(float(v)) for v in s], mp.mpf(pt), cfg.spin, 1)
+        assert abs(float(Hv)) < 1e-13
+        assert pt < 0  # future-directed: E = -p_t > 0
+
+
+# ------------------------------------------------------- DP5(4) stepper
+def _scipy_tableau():
+    from scipy.integrate._ivp.rk import RK45
+    return RK45.A, RK45.B, RK45.C, RK45.E
+
+
+def test_dp_attempt_matches_scipy_tableau():
+    A, B, C, E = _scipy_tableau()
+    cfg, _ = K.preset("cpu256_a09")
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        s = np.concatenate([rng.uniform(-15, 15, 3), rng.normal(size=3)])
+        pt = -1.1
+        h = rng.uniform(1.0, 4.0)
+        tol = 1e-8
+        k, ns, err = O.dp_attempt(cfg, s, pt, h, tol)
+        f = lambda y: O.get_derivs(cfg, y, pt)  # noqa: E731
+        K_ = np.zeros((7, 6))
+        K_[0] = f(s)
+        for i in range(1, 6):
+            K_[i] = f(s + h * (A[i, :i] @ K_[:i]))
+        y_new = s + h * (B @ K_[:6])
+        K_[6] = f(y_new)
+        scale = 1e-8 * np.maximum(np.abs(s), 1.0)
+        err_ref = np.max(np.abs(h * (E @ K_)) / scale)
+        np.testing.assert_allclose(k, K_, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(ns, y_new, rtol=1e-14, atol=1e-14)
+        # the error norm is a heavily cancelling sum; agreement is limited
+        # by its rounding (ulp of the largest term / the sum)
+        assert err == pytest.approx(err_ref, rel=1e-3)
+
+
+def test_tableau_consistency():
+    A, B, C, E = _scipy_tableau()
+    np.testing.assert_allclose(A.sum(axis=1), C, atol=1e-15)
+    assert B.sum() == pytest.approx(1.0, abs=1e-15)
+    # E sums to zero (difference of two weight vectors of sum one)
+    assert abs(E.sum()) < 1e-15
+
+
+def _dense_pieces(cfg, s, pt, h):
+    k, ns, _ = O.dp_attempt(cfg, s, pt, h, 1e-8)
+    return k, ns
+
+
+def test_dense_output_endpoints_and_order():
+    cfg, _ = K.preset("cpu256_a09")
+    s = np.array([12.0, 3.0, -4.0, -0.6, -0.3, 0.5])
+    pt = -1.0
+
+    def dense_state(k, s0, s1, h, t, der=False):
+        return np.array([O.dense(s0[i], s1[i], k[0, i], k[2, i], k[3, i], k[4, i], k[5, i], k[6, i], h, t,
+                                 derivative=der) for i in range(6)])
+
+    h = 0.8
+    k, ns = _dense_pieces(cfg, s, pt, h)
+    np.testing.assert_allclose(dense_state(k, s, ns, h, 0.0), s, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(dense_state(k, s, ns, h, 1.0), ns, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(dense_state(k, s, ns, h, 0.0, True), h * k[0], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(dense_state(k, s, ns, h, 1.0, True), h * k[6], rtol=1e-12, atol=1e-14)
+
+    # 4th-order interpolant: local error at t = 0.5 scales like h^5
+    def fine(y, total, n=400):
+        dh = total / n
+        for _ in range(n):
+            _, y = _dense_pieces(cfg, y, pt, dh)
+        return y
+
+    errs = []
+    for hh in (1.6, 0.8, 0.4):
+        k, ns = _dense_pieces(cfg, s, pt, hh)
+        mid = dense_state(k, s, ns, hh, 0.5)
+        errs.append(np.max(np.abs(mid - fine(s, 0.5 * hh))))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 20 < r1 < 48 and 20 < r2 < 48, errs
+
+
+# ------------------------------------------------------ trajectories
+def _no_disk_cfg(a=0.0, R=1000.0, tol=1e-10):
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.spin = a
+    cfg.disk_peak_temperature = 0.0
+    cfg.disk_temperature_scale = 0.0
+    cfg.escape_radius = R
+    cfg.tolerance = tol
+    cfg.max_steps = 20000
+    return cfg
+
+
+@pytest.mark.parametrize("b,expect", [(5.0, 1), (5.15, 1), (5.25, 2), (5.5, 2)])
+def test_schwarzschild_capture_threshold(b, expect):
+    """Capture iff impact parameter b < sqrt(27) M = 5.196 M."""
+    D = 1000.0
+    cfg = _no_disk_cfg(0.0, D)
+    alpha = math.asin(b / D)
+    col, cls, st, _ = O.trace_dir(cfg, [D, 0.0, 0.0], [-math.cos(alpha), math.sin(alpha), 0.0])
+    assert (cls & 3) == expect
+
+
+def test_weak_deflection_angle():
+    """Deflection 4M/b + 15 pi M^2 / (4 b^2) for b >> M."""
+    D, b = 1e6, 200.0
+    cfg = _no_disk_cfg(0.0, D, 1e-11)
+    alpha = math.asin(b / D)
+    d0 = np.array([-math.cos(alpha), math.sin(alpha), 0.0])
+    col, cls, st, log = O.trace_dir(cfg, [D, 0.0, 0.0], d0, max_log=50000)
+    assert (cls & 3) == 2
+    p_end = log[-1, 5:8]
+    p_end = p_end / np.linalg.norm(p_end)
+    defl = math.acos(float(np.clip(np.dot(p_end, d0), -1, 1)))
+    expect = 4.0 / b + 15 * math.pi / (4 * b * b)
+    assert defl == pytest.approx(expect, rel=2e-3)
+
+
+def _ham_np(s, pt, a, M=1.0):
+    x, y, z, px, py, pz = s
+    u = x * x + y * y + z * z - a * a
+    r = math.sqrt((u + math.sqrt(u * u + 4 * a * a * y * y)) / 2)
+    den = r * r + a * a
+    lx, ly, lz = (r * x - a * z) / den, y / r, (r * z + a * x) / den
+    H = M * r**3 / (r**4 + a * a * y * y)
+    lp = lx * px + ly * py + lz * pz - pt
+    return 0.5 * (px * px + py * py + pz * pz - pt * pt) - H * lp * lp
+
+
+@pytest.mark.parametrize("a", [0.0, 0.9, 0.998])
+def test_null_condition_conserved_along_rays(a):
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg.spin = a
+    rng = np.random.default_rng(6)
+    for _ in range(6):
+        d = np.array(cam.fwd) + rng.uniform(-0.08, 0.08) * np.array(cam.rgt) + rng.uniform(-0.05, 0.05) * np.array(cam.up)
+        s0, pt, _ = O.init_ray(cfg, list(cam.pos), d)
+        _, cls, st, log = O.trace_dir(cfg, list(cam.pos), d, max_log=20000)
+        assert len(log) > 3
+        drift = max(abs(_ham_np(row[2:8], pt, a)) for row in log)
+        assert drift < 1e-6, (cls, drift)
+
+
+def test_tighter_tolerance_converges():
+    """Exit directions converge as the tolerance tightens (escape sphere far
+    enough that the bending left at exit is negligible)."""
+    cfg, cam = K.preset("cpu256_a09")
+    cfg.disk_peak_temperature = 0.0
+    cfg.disk_temperature_scale = 0.0
+    cfg.escape_radius = 1e5
+    d = np.array(cam.fwd) + 0.1 * np.array(cam.rgt) + 0.05 * np.array(cam.up)
+    ends = []
+    for tol in (1e-7, 1e-9, 1e-11):
+        cfg.tolerance = tol
+        _, cls, _, log = O.trace_dir(cfg, list(cam.pos), d, max_log=20000)
+        assert (cls & 3) == 2
+        ends.append(log[-1, 5:8] / np.linalg.norm(log[-1, 5:8]))
+    e01, e12 = np.linalg.norm(ends[0] - ends[2]), np.linalg.norm(ends[1] - ends[2])
+    assert e12 < e01
+    assert e12 < 1e-6
+
+
+def test_reference_rhs_differs_only_in_dpy():
+    """The as-written right-hand side differs from the exact Hamiltonian
+    gradient only in dp_y, and not at all on the equator (dr^2/dy ~ y)."""
+    cfg, _ = K.preset("cpu256_a09")
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        s = np.concatenate([rng.uniform(-10, 10, 3), rng.normal(size=3)])
+        cfg.rhs_mode = K.KT_RHS_HAMILTONIAN
+        exact = O.get_derivs(cfg, s, -1.0)
+        cfg.rhs_mode = K.KT_RHS_REFERENCE
+        ref = O.get_derivs(cfg, s, -1.0)
+        np.testing.assert_array_equal(np.delete(exact, 4), np.delete(ref, 4))
+        assert ref[4] != exact[4]
+        s[1] = 0.0
+        cfg.rhs_mode = K.KT_RHS_HAMILTONIAN
+        e0 = O.get_derivs(cfg, s, -1.0)
+        cfg.rhs_mode = K.KT_RHS_REFERENCE
+        np.testing.assert_array_equal(e0, O.get_derivs(cfg, s, -1.0))
+
+
+# ------------------------------------------------------------- disk LUT
+def _page_thorne_terms(r, a):
+    """Page-Thorne flux factor pieces (M = 1): x = sqrt(r); roots x_i of
+    x^3 - 3x + 2a = 0; per-root log terms."""
+    x = mp.sqrt(mp.mpf(r))
+    base = mp.acos(-mp.mpf(a)) / 3
+    roots = [2 * mp.cos(base), 2 * mp.cos(base - 2 * mp.pi / 3), 2 * mp.cos(base + 2 * mp.pi / 3)]
+    return x, roots
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9, 0.998, -0.45])
+def test_nt_factor_is_page_thorne_minus_dropped_terms(a):
+    """The reference (utils.rs:21-52) keeps only log terms whose root
+    denominator is positive; adding the dropped terms back must give the
+    textbook Page-Thorne factor exactly."""
+    r_isco = O.isco(abs(a))
+    a_m = mp.mpf(a)
+    x0, roots = _page_thorne_terms(r_isco, a)
+    for r in np.linspace(r_isco * 1.01, 30.0, 7):
+        x, _ = _page_thorne_terms(r, a)
+        full = x - x0 - mp.mpf(1.5) * a_m * mp.log(x / x0)
+        kept = full
+        for i in range(3):
+            xi = roots[i]
+            den = xi * (xi - roots[(i + 1) % 3]) * (xi - roots[(i + 2) % 3])
+            coef = 3 * (xi - a_m) ** 2 / den
+            term = coef * mp.log((x - xi) / (x0 - xi)) if (x - xi) / (x0 - xi) > 0 else 0
+            full -= term
+            if den > 0:
+                kept -= term
+        geo = r * (x**3 - 3 * x + 2 * a_m)
+        got = O.nt_factor(float(r), a, r_isco, 1.0)
+        assert got == pytest.approx(max(float(kept / geo), 0.0), rel=1e-10, abs=1e-15)
+        if a > 0:
+            # the textbook factor is positive outside the ISCO
+            assert float(full / geo) > 0
+    assert O.nt_factor(r_isco, a, r_isco, 1.0) == 0.0
+
+
+def test_disk_lut_layout_and_peak_normalisation():
+    cfg, _ = K.preset("cpu256_a09")
+    d, disk, bb = O.derive(cfg)
+    n = cfg.lut_size
+    assert disk[0] == 0.0  # r = ISCO
+    r_i = d.disk_inner + np.arange(n) / (n - 1) * (d.disk_outer - d.disk_inner)
+    for i in (1, 17, 900, n - 1):
+        f = O.nt_factor(float(r_i[i]), 0.9, d.disk_inner, 1.0)
+        assert disk[i] == pytest.approx(f ** 0.25, rel=1e-13)
+    assert d.temperature_scale * disk.max() == pytest.approx(1e4, rel=1e-14)
+    cfg.mdot_scale = 0.8
+    d2, _, _ = O.derive(cfg)
+    assert d2.temperature_scale == pytest.approx(d.temperature_scale * 0.8 ** 0.25, rel=1e-15)
+
+
+# ------------------------------------------------------ blackbody LUT
+# CIE 1931 2-degree chromaticity of the Planckian locus (tabulated values)
+PLANCK_XY = {2000: (0.5267, 0.4133), 3000: (0.4369, 0.4041), 4000: (0.3805, 0.3768),
+             5000: (0.3451, 0.3516), 6500: (0.3135, 0.3237), 10000: (0.2807, 0.2884)}
+SRGB_TO_XYZ = np.array([[0.4124564, 0.3575761, 0.1804375],
+                        [0.2126729, 0.7151522, 0.0721750],
+                        [0.0193339, 0.1191920, 0.9503041]])
+
+
+def test_blackbody_lut_physics():
+    cfg, _ = K.preset("cpu256_a09")
+    _, _, bb = O.derive(cfg)
+    n = cfg.lut_size
+    assert np.all(bb[0] == 0.0)
+    # luminance (Rec.709 weights on linear sRGB) grows with temperature
+    lum = bb @ np.array([0.2126, 0.7152, 0.0722])
+    assert np.all(np.diff(lum[1:]) >= 0)
+    # the LUT colour lies on the Planckian locus: back to XYZ with the
+    # inverse (IEC 61966-2-1) matrix, chromaticity within 0.006 of the
+    # tabulated locus (the analytic CMF fit and the 10 nm sampling)
+    for T, (x0, y0) in PLANCK_XY.items():
+        i = int(round(T / cfg.lut_max_temp * (n - 1)))
+        X, Y, Z = SRGB_TO_XYZ @ bb[i]
+        x, y = X / (X + Y + Z), Y / (X + Y + Z)
+        assert abs(x - x0) < 0.006 and abs(y - y0) < 0.006, (T, x, y)
+    # LUT entry i holds temperature i/(N-1) * T_max: a 6504 K entry is near white
+    i65 = int(round(6504 / cfg.lut_max_temp * (n - 1)))
+    c = bb[i65] / bb[i65].mean()
+    assert np.all(np.abs(c - 1) < 0.12), c
+
+
+# ------------------------------------------------------- noise
+def test_noise_lattice_zero_and_fbm_flat():
+    for ix, iy in [(0, 0), (3, -7), (-12, 40)]:
+        assert O.gradient_noise(float(ix), float(iy), 0) == 0.0
+        assert O.gradient_noise(float(ix), float(iy), 5) == 0.0
+    assert O.fbm(1.3, 2.7, 0, 0) == 0.5
+    vals = [O.fbm(u, v, 6, 1) for u, v in np.random.default_rng(7).uniform(-50, 50, (500, 2))]
+    assert 0.0 <= min(vals) and max(vals) <= 1.0
+
+
+def test_noise_gradient_matches_hash():
+    """Near a lattice point the noise is g . (u - ix), g = (cos, sin)(2 pi h / 2^32)."""
+    for seed in (0, 1):
+        mix = (seed * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)
+        ix, iy = 5, -3
+        h = _pcg32_bigint(((ix & 0xFFFFFFFF) << 32 | (iy & 0xFFFFFFFF)) ^ mix)
+        ang = h * (2 * math.pi * 2.0**-32)
+        e = 1e-7
+        gu = O.gradient_noise(ix + e, iy, seed) / e
+        gv = O.gradient_noise(ix, iy + e, seed) / e
+        assert gu == pytest.approx(math.cos(ang), abs=1e-5)
+        assert gv == pytest.approx(math.sin(ang), abs=1e-5)
+
+
+# ---------------------------------------------------- post processing
+def _brute_bloom(h3, R, sigma):
+    from scipy.ndimage import correlate1d
+    i = np.arange(-R, R + 1)
+    w = np.exp(-(i * i) / (2 * sigma * sigma))
+    ones = np.ones(h3.shape[:2])
+    hn = correlate1d(h3, w, axis=1, mode="constant") / correlate1d(ones, w, axis=1, mode="constant")[..., None]
+    vn = correlate1d(hn, w, axis=0, mode="constant") / correlate1d(ones, w, axis=0, mode="constant")[..., None]
+    return vn
+
+
+def test_bloom_is_renormalised_truncated_gaussian():
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.width, cfg.height = 97, 61
+    cfg.bloom_threshold = 0.0
+    rng = np.random.default_rng(8)
+    h = np.ones((61, 97, 4), dtype=np.float32)
+    h[..., :3] = rng.random((61, 97, 3)).astype(np.float32) * 3
+    _, bloom = O.post(cfg, h)
+    ref = _brute_bloom(h[..., :3].astype(np.float64), 24, 8.0)
+    np.testing.assert_allclose(bloom, ref, rtol=1e-12, atol=1e-13)
+    # the bright-pass threshold subtracts before blurring
+    cfg.bloom_threshold = 1.0
+    _, bloom_t = O.post(cfg, h)
+    ref_t = _brute_bloom(np.maximum(h[..., :3].astype(np.float64) - 1.0, 0.0), 24, 8.0)
+    np.testing.assert_allclose(bloom_t, ref_t, rtol=1e-12, atol=1e-13)
+
+
+def test_display_transform_known_points():
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.width, cfg.height = 6, 1
+    cfg.bloom_enabled = 0
+    vals = [0.0, 1e-6, 0.18, 1.0, 10.0, 1e6]
+    h = np.ones((1, 6, 4), dtype=np.float32)
+    for i, v in enumerate(vals):
+        h[0, i, :3] = v
+    ldr, _ = O.post(cfg, h)
+    got = (ldr[0] & 255).tolist()
+
+    def expect(x):
+        x = float(np.float32(x))
+        y = x * (2.51 * x + 0.03) / (x * (2.43 * x + 0.59) + 0.14)  # Narkowicz 2015
+        s = 12.92 * y if y <= 0.0031308 else 1.055 * y ** (1 / 2.4) - 0.055  # IEC 61966-2-1
+        return int(np.rint(min(max(s, 0.0), 1.0) * 255))
+
+    assert got == [expect(v) for v in vals]
+    assert got[0] == 0 and got[-1] == 255
+    assert (ldr[0] >> 16).tolist() == got and ((ldr[0] >> 8) & 255).tolist() == got
+
+
+# ---------------------------------------------------- whole frame
+def test_frame_classes_and_symmetry():
+    """Face-on (i = 0) Schwarzschild frame: rotationally symmetric shadow of
+    angular radius asin(sqrt(27) M / r_obs * sqrt(1 - 2M/r_obs))."""
+    cfg, cam = K.preset("cpu256_a09")
+    cfg.spin = 0.0
+    cfg.disk_peak_temperature = 0.0
+    cfg.disk_temperature_scale = 0.0
+    cfg.width = cfg.height = 65
+    cam = K.observer_camera(50.0, 0.0, 30.0)
+    # closed-form camera: the basis must be orthonormal
+    B = np.array([cam.fwd, cam.rgt, cam.up])
+    np.testing.assert_allclose(B @ B.T, np.eye(3), atol=1e-15)
+    hdr, cls, att, st = O.render(cfg, cam)
+    hc = cls & 3
+    frac = (hc == 1).mean()
+    r_obs = 50.0
+    # shadow edge in the local static frame (Synge 1966)
+    sin_psi = math.sqrt(27.0) / r_obs * math.sqrt(1 - 2 / r_obs)
+    tan_half_fov = math.tan(math.radians(15.0))
+    shadow_r_px = math.tan(math.asin(sin_psi)) / tan_half_fov * 32.5
+    expect = math.pi * shadow_r_px**2 / 65**2
+    assert frac == pytest.approx(expect, rel=0.1)
+    # four-fold symmetry of the class image (jitter aside, up to the edge)
+    mism = (hc != np.rot90(hc)).mean()
+    assert mism < 0.03
