@@ -87,7 +87,8 @@ enum {
     KT_END_ESCAPE = 2,      /* r > escape_radius (kerr_trace.cuh:229-237) */
     KT_FLAG_DISK = 4,       /* at least one disk emission was composited */
     KT_FLAG_SKY_GRID = 8,   /* the escape direction fell on a sky grid line */
-    KT_FLAG_LUT_OVERFLOW = 16 /* a disk temperature exceeded lut_max_temp */
+    KT_FLAG_LUT_OVERFLOW = 16, /* a disk temperature exceeded lut_max_temp */
+    KT_DISK_COUNT_SHIFT = 5   /* bits 5-7: disk contributions, saturating at 7 */
 };
 
 /*

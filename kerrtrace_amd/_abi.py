@@ -28,6 +28,7 @@ KT_END_ESCAPE = 2
 KT_FLAG_DISK = 4
 KT_FLAG_SKY_GRID = 8
 KT_FLAG_LUT_OVERFLOW = 16
+KT_DISK_COUNT_SHIFT = 5
 
 
 class KtConfig(C.Structure):

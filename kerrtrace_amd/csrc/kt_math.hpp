@@ -152,6 +152,89 @@ __device__ __forceinline__ V6 rhs(const V6& s, double pt, const Geo& g) {
     return d;
 }
 
+// Mixed-precision right-hand side.  The Kerr-Schild radius is solved in
+// FP64 from the FP64 position (u + sqrt(u^2 + 4a^2y^2) cancels for u < 0
+// near the ring); everything after it runs in FP32.  The result is split
+// as dx/dl = p - corr, dp/dl = kp, with corr = 2H(l.p - pt) l: the
+// free-streaming part p stays with the FP64 state and only the
+// gravitational pieces corr and kp (both O(M/r)) carry FP32 rounding.
+struct G32 {
+    float a, aa, M;
+    bool exact;
+};
+struct K32 {
+    float cx, cy, cz;    // corr
+    float px, py, pz;    // dp/dl
+};
+__device__ __forceinline__ K32 rhs_mixed(double x, double y, double z, float pxf, float pyf, float pzf, float ptf,
+                                         double aa_d, const G32& g) {
+    // FP64 radius solve
+    double rho2 = fma(x, x, fma(y, y, z * z));
+    double u = rho2 - aa_d;
+    double s_sq = fma(u, u, 4.0 * aa_d * y * y);
+    double s = sqrt(s_sq);
+    double r2d = 0.5 * (u + s);
+    // FP32 bulk
+    const float xf = (float)x, yf = (float)y, zf = (float)z;
+    const float a2 = g.aa;
+    const float uf = (float)u;
+    const float inv_s = __builtin_amdgcn_rcpf((float)s);
+    const float r2 = (float)r2d;
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * inv_r;
+    float dr2_dx = xf * (1.0f + uf * inv_s);
+    float dr2_dy = g.exact ? yf * (1.0f + (uf + 2.0f * a2) * inv_s) : yf * (1.0f + (uf + 4.0f * a2) * (0.5f * inv_s));
+    float dr2_dz = zf * (1.0f + uf * inv_s);
+    float inv_2r = 0.5f * inv_r;
+    float drdx = dr2_dx * inv_2r, drdy = dr2_dy * inv_2r, drdz = dr2_dz * inv_2r;
+    float denom = r2 + a2;
+    float inv_denom = __builtin_amdgcn_rcpf(denom);
+    float inv_denom2 = inv_denom * inv_denom;
+    float Nx = r * xf - g.a * zf;
+    float Nz = r * zf + g.a * xf;
+    float lx = Nx * inv_denom;
+    float ly = yf * inv_r;
+    float lz = Nz * inv_denom;
+    float r3 = r2 * r;
+    float denomH = fmaf(r2, r2, a2 * yf * yf);
+    float inv_denomH = __builtin_amdgcn_rcpf(denomH);
+    float H = g.M * r3 * inv_denomH;
+    float lp = fmaf(lx, pxf, fmaf(ly, pyf, fmaf(lz, pzf, -ptf)));
+    float c = 2.0f * H * lp;
+    K32 k;
+    k.cx = c * lx;
+    k.cy = c * ly;
+    k.cz = c * lz;
+    float dD_dx = 2.0f * r * drdx, dD_dy = 2.0f * r * drdy, dD_dz = 2.0f * r * drdz;
+    float dlx_dx = ((drdx * xf + r) * denom - Nx * dD_dx) * inv_denom2;
+    float dlx_dy = ((drdy * xf) * denom - Nx * dD_dy) * inv_denom2;
+    float dlx_dz = ((drdz * xf - g.a) * denom - Nx * dD_dz) * inv_denom2;
+    float dlz_dx = ((drdx * zf + g.a) * denom - Nz * dD_dx) * inv_denom2;
+    float dlz_dy = ((drdy * zf) * denom - Nz * dD_dy) * inv_denom2;
+    float dlz_dz = ((drdz * zf + r) * denom - Nz * dD_dz) * inv_denom2;
+    float inv_r2 = inv_r * inv_r;
+    float dly_dx = -yf * drdx * inv_r2;
+    float dly_dy = (r - yf * drdy) * inv_r2;
+    float dly_dz = -yf * drdz * inv_r2;
+    float f4r3 = 4.0f * r3;
+    float dDenH_dx = f4r3 * drdx;
+    float dDenH_dy = fmaf(f4r3, drdy, 2.0f * a2 * yf);
+    float dDenH_dz = f4r3 * drdz;
+    float factor_H = g.M * inv_denomH * inv_denomH;
+    float t3r2 = 3.0f * r2;
+    float dH_dx = factor_H * (t3r2 * drdx * denomH - r3 * dDenH_dx);
+    float dH_dy = factor_H * (t3r2 * drdy * denomH - r3 * dDenH_dy);
+    float dH_dz = factor_H * (t3r2 * drdz * denomH - r3 * dDenH_dz);
+    float dlp_dx = dlx_dx * pxf + dly_dx * pyf + dlz_dx * pzf;
+    float dlp_dy = dlx_dy * pxf + dly_dy * pyf + dlz_dy * pzf;
+    float dlp_dz = dlx_dz * pxf + dly_dz * pyf + dlz_dz * pzf;
+    float lp2 = lp * lp;
+    k.px = dH_dx * lp2 + c * dlp_dx;
+    k.py = dH_dy * lp2 + c * dlp_dy;
+    k.pz = dH_dz * lp2 + c * dlp_dz;
+    return k;
+}
+
 // DOPRI5 continuous extension (cuda/kerr_math.cuh:186-219; Hairer, Norsett
 // and Wanner's dense-output coefficients).
 struct Dense {

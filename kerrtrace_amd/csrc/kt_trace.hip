@@ -24,331 +24,9 @@
 
 #include "kt_internal.hpp"
 #include "kt_math.hpp"
+#include "kt_tracer_common.hpp"
 
 namespace kt {
-
-// DP5(4) tableau (cuda/kerr_trace.cuh:52-59, docs/dormand_prince_extended_tableau.md)
-#define KT_A21 (1.0 / 5.0)
-#define KT_A31 (3.0 / 40.0)
-#define KT_A32 (9.0 / 40.0)
-#define KT_A41 (44.0 / 45.0)
-#define KT_A42 (-56.0 / 15.0)
-#define KT_A43 (32.0 / 9.0)
-#define KT_A51 (19372.0 / 6561.0)
-#define KT_A52 (-25360.0 / 2187.0)
-#define KT_A53 (64448.0 / 6561.0)
-#define KT_A54 (-212.0 / 729.0)
-#define KT_A61 (9017.0 / 3168.0)
-#define KT_A62 (-355.0 / 33.0)
-#define KT_A63 (46732.0 / 5247.0)
-#define KT_A64 (49.0 / 176.0)
-#define KT_A65 (-5103.0 / 18656.0)
-#define KT_B1 (35.0 / 384.0)
-#define KT_B3 (500.0 / 1113.0)
-#define KT_B4 (125.0 / 192.0)
-#define KT_B5 (-2187.0 / 6784.0)
-#define KT_B6 (11.0 / 84.0)
-#define KT_E1 (35.0 / 384.0 - 5179.0 / 57600.0)
-#define KT_E3 (500.0 / 1113.0 - 7571.0 / 16695.0)
-#define KT_E4 (125.0 / 192.0 - 393.0 / 640.0)
-#define KT_E5 (-2187.0 / 6784.0 + 92097.0 / 339200.0)
-#define KT_E6 (11.0 / 84.0 - 187.0 / 2100.0)
-#define KT_E7 (-1.0 / 40.0)
-
-static constexpr int kWave = 64;
-
-// Potential-scaled tolerance of the step that starts at s
-// (cuda/kerr_trace.cuh:67-73, with the tolerance floor extension).
-__device__ __forceinline__ double step_tolerance(const V6& s, const Geo& g, double tol, double tol_floor) {
-    double r_c = ks_r(s.x, s.y, s.z, g.aa);
-    double r2_c = r_c * r_c;
-    double dH = fma(r2_c, r2_c, g.aa * s.y * s.y);
-    double Hc = g.M * r_c * r2_c * (1.0 / dH);
-    double tq = (Hc - 0.05) / (0.3 - 0.05);
-    tq = fmin(fmax(tq, 0.0), 1.0);
-    return tol * fmax(5.0 - 5.0 * tq, tol_floor);
-}
-
-// Blackbody colour of temperature T (cuda/kernel.cu:1-16); the texture's
-// clamped linear filter is done with exact weights at texel coordinate
-// T / T_max * (N - 1).
-__device__ __forceinline__ void blackbody(const RenderParams& P, double T, double rgb[3], bool& overflow) {
-    if (T > P.lut_max_temp) {
-        overflow = true;
-        rgb[0] = rgb[1] = rgb[2] = 0.0;
-        return;
-    }
-    double t_norm = T / P.lut_max_temp;
-    double pos = t_norm * (double)(P.lut_n - 1);
-    double fl = floor(pos);
-    int i0 = (int)fl;
-    double w = pos - fl;
-    int i1 = i0 + 1;
-    i0 = min(max(i0, 0), P.lut_n - 1);
-    i1 = min(max(i1, 0), P.lut_n - 1);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double c0 = P.bb_lut[i0 * 3 + c], c1 = P.bb_lut[i1 * 3 + c];
-        rgb[c] = c0 + w * (c1 - c0);
-    }
-}
-
-// Disk flux LUT at u in [0,1] (cuda/kerr_trace.cuh:196-198): normalised
-// texture coordinate without a half-texel offset, i.e. texel u*N - 0.5.
-__device__ __forceinline__ double disk_factor(const RenderParams& P, double u) {
-    double xb = u * (double)P.lut_n - 0.5;
-    double fl = floor(xb);
-    int i0 = (int)fl;
-    double w = xb - fl;
-    int i1 = i0 + 1;
-    i0 = min(max(i0, 0), P.lut_n - 1);
-    i1 = min(max(i1, 0), P.lut_n - 1);
-    double c0 = P.disk_lut[i0], c1 = P.disk_lut[i1];
-    return c0 + w * (c1 - c0);
-}
-
-// Emission of one equatorial crossing (cuda/kerr_trace.cuh:174-226).
-// Returns true when colour was composited.
-__device__ __noinline__ bool shade_crossing(const RenderParams& P, double x_hit, double z_hit, double px_hit,
-                                            double py_hit, double pz_hit, double pt, double kt, double col[3],
-                                            double& Tr, bool& overflow, double& overflow_T) {
-    const double a = P.a, M = P.M, aa = P.aa;
-    double r_hit = ks_r(x_hit, 0.0, z_hit, aa);
-    if (!(r_hit >= P.disk_inner && r_hit <= P.disk_outer)) return false;
-    double sqrt_M = sqrt(M), r_sqrt = sqrt(r_hit);
-    double disc_denom = 1.0 - 3.0 * M / r_hit + 2.0 * a * sqrt_M / (r_hit * r_sqrt);
-    if (!(disc_denom > 0.0)) return false;
-    double omega = -sqrt_M / (r_hit * r_sqrt + a * sqrt_M);
-    double r2_hit = r_hit * r_hit;
-    double denom_hit = r2_hit + aa;
-    double inv_dh = 1.0 / denom_hit;
-    double hlx = (r_hit * x_hit - a * z_hit) * inv_dh;
-    double hly = 0.0;
-    double hlz = (r_hit * z_hit + a * x_hit) * inv_dh;
-    double ldotp = fma(hlx, px_hit, fma(hly, py_hit, hlz * pz_hit));
-    double ldotxi = hlx * z_hit - hlz * x_hit;
-    double dHh = r2_hit * r2_hit;
-    double Hh = M * r_hit * r2_hit * (1.0 / dHh);
-    double L = (z_hit * px_hit - x_hit * pz_hit) + 2.0 * Hh * ldotxi * (ldotp + kt);
-    double energy = -pt;
-    double l_over_e = L / energy;
-    double g = 1.0 / ((1.0 / sqrt(disc_denom)) * (1.0 - omega * l_over_e));
-    double dspan = P.disk_outer - P.disk_inner;
-    double uu = dspan > 0.0 ? (r_hit - P.disk_inner) / dspan : 0.0;
-    double sampled = disk_factor(P, uu);
-    double T = P.T_scale * sampled * g;
-    if (P.noise_detail > 0) {
-        // Boyer-Lindquist phi of the hit (cuda/kerr_math.cuh:60-81; r_hit is
-        // overwritten with r2 * inv_r as in kerr_trace.cuh:203)
-        double rho2 = fma(x_hit, x_hit, fma(0.0, 0.0, z_hit * z_hit));
-        double u = rho2 - aa;
-        double s_sq = fma(u, u, 4.0 * aa * 0.0 * 0.0);
-        double inv_s = 1.0 / sqrt(s_sq);
-        double s = s_sq * inv_s;
-        double r2 = 0.5 * (u + s);
-        double inv_r = 1.0 / sqrt(r2);
-        r_hit = r2 * inv_r;
-        double cos_th = 0.0 * inv_r;
-        double cos2 = cos_th * cos_th;
-        double sin_sq = 1.0 - cos2;
-        double inv_sin = 1.0 / sqrt(sin_sq);
-        double denom = r2 + aa;
-        double inv = inv_sin * (1.0 / denom);
-        double cos_phi = (r_hit * z_hit + a * x_hit) * inv;
-        double sin_phi = (r_hit * x_hit - a * z_hit) * inv;
-        double phi = atan2(sin_phi, cos_phi);
-        double r3 = r_hit * r_hit * r_hit;
-        double om = 1.0 / (a + sqrt(r3 * P.inv_M));
-        double phase = phi + P.noise_winding * om;
-        double sn, cs;
-        sincos(phase, &sn, &cs);
-        double nu = r_hit * cs * P.noise_scale;
-        double nv = r_hit * sn * P.noise_scale;
-        double nval = fbm(nu, nv, P.noise_detail, P.noise_seed_mix);
-        double modulator = 1.0 + P.noise_strength * (nval - 0.5);
-        T = T * modulator;
-    }
-    if (!(__builtin_isfinite(T) && T > 0.0)) return false;
-    double d_col[3];
-    bool ovf = false;
-    blackbody(P, T, d_col, ovf);
-    if (ovf && !overflow) {
-        overflow = true;
-        overflow_T = T;
-    }
-    double luma = 0.2126 * d_col[0] + 0.7152 * d_col[1] + 0.0722 * d_col[2];
-    double luma_scaled = luma / P.wl_step;
-    double alpha = 1.0 - exp(-luma_scaled * 0.5);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) col[c] = col[c] + d_col[c] * alpha * Tr;
-    Tr = Tr * (1.0 - alpha);
-    return alpha > 0.0;
-}
-
-// Sky grid intensity along the escape direction (cuda/kerr_math.cuh:157-171).
-__device__ __noinline__ double sky(const RenderParams& P, const V6& s, double pt, double Tr, bool& grid) {
-    Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
-    V6 ke = rhs(s, pt, g);
-    double q = ke.x * ke.x + ke.y * ke.y + ke.z * ke.z;
-    double inv = 1.0 / sqrt(q);
-    double dx = ke.x * inv, dy = ke.y * inv, dz = ke.z * inv;
-    double theta = atan2(dz, dx);
-    double phi_lat = asin(dy);
-    double spacing = P.sky_spacing;
-    double lt = P.sky_lt;
-    double d_theta = fmod(fabs(theta), spacing);
-    double d_phi = fmod(fabs(phi_lat), spacing);
-    grid = (d_theta < lt || d_theta > (spacing - lt)) || (d_phi < lt || d_phi > (spacing - lt)) ||
-           (fabs(phi_lat) < lt * 3.0);
-    return grid ? P.sky_int * Tr : 0.0;
-}
-
-struct LaneCounters {
-    uint32_t rhs = 0, attempts = 0, accepted = 0, crossings = 0, max_att = 0;
-};
-
-// Decode a global work item into (render, pixel).  Items past the end of a
-// render's tiled range or outside its image return false.
-// The parameter record of render rid (always record 0 for a single render,
-// so its loads are wave-uniform scalar loads).
-template <bool BATCH>
-__device__ __forceinline__ const RenderParams& prm(const RenderParams* __restrict__ Ps, int rid) {
-    return BATCH ? Ps[rid] : Ps[0];
-}
-
-template <bool BATCH>
-__device__ __forceinline__ bool decode_item(const RenderParams* __restrict__ Ps, int n_renders, uint64_t item,
-                                            int& rid, int& px, int& py) {
-    rid = 0;
-    if (BATCH) {
-        int lo = 0, hi = n_renders - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (Ps[mid].ray_begin <= item) lo = mid; else hi = mid - 1;
-        }
-        rid = lo;
-    }
-    const RenderParams& P = prm<BATCH>(Ps, rid);
-    uint64_t local = item - P.ray_begin;
-    if (local >= P.ray_count) return false;
-    uint32_t T = (uint32_t)P.tile;
-    uint32_t tt = T * T;
-    uint32_t tile_id = (uint32_t)(local / tt);
-    uint32_t within = (uint32_t)(local - (uint64_t)tile_id * tt);
-    uint32_t ty = tile_id / (uint32_t)P.tiles_x;
-    uint32_t tx = tile_id - ty * (uint32_t)P.tiles_x;
-    px = (int)(tx * T + within % T);
-    py = (int)(ty * T + within / T);
-    return px < P.W && py < P.H;
-}
-
-
-// Per-lane ray state of the persistent tracer.
-struct Lane {
-    V6 s, k1;
-    double pt, kt, h, tol, Tr;
-    double col[3], acc[3];
-    double overflow_T;
-    int rid, pix_x, pix_y, sample, step, att_in_step, cls;
-    uint32_t ray_att;
-    bool active, have_k1, overflow;
-};
-
-// Hot per-render values a lane keeps in registers.
-struct Hot {
-    Geo g;
-    double tol, tol_floor, cutoff, rh_eps, R_esc;
-    int max_steps, max_attempts, spp;
-};
-
-__device__ __forceinline__ Hot load_hot(const RenderParams& P) {
-    Hot o;
-    o.g = Geo{P.a, P.aa, P.M, P.rhs_exact != 0};
-    o.tol = P.tol;
-    o.tol_floor = P.tol_floor;
-    o.cutoff = P.cutoff;
-    o.rh_eps = P.rh_eps;
-    o.R_esc = P.R_esc;
-    o.max_steps = P.max_steps;
-    o.max_attempts = P.max_attempts;
-    o.spp = P.spp;
-    return o;
-}
-
-// Initial state of sample L.sample of the lane's pixel: jittered pinhole
-// direction (cuda/kernel.cu:112-121) and the null momentum in Kerr-Schild
-// form (cuda/kerr_trace.cuh:16-47).
-__device__ __forceinline__ void start_sample(Lane& L, const RenderParams& P, const Hot& H) {
-    const Geo& g = H.g;
-    uint32_t pixel_index = (uint32_t)(L.pix_y * P.W + L.pix_x);
-    uint64_t base = ((uint64_t)pixel_index << 32) | (uint64_t)(uint32_t)L.sample;
-    double rx = (double)pcg32(base) * 0x1p-32;
-    double ry = (double)pcg32(base ^ 0xda3e39cb94b95bdbULL) * 0x1p-32;
-    double u = (((double)L.pix_x + rx) * P.inv_w_2 - 1.0) * P.aspect * P.fov_scale;
-    double v = (1.0 - ((double)L.pix_y + ry) * P.inv_h_2) * P.fov_scale;
-    double dx = u * P.rgt[0] + v * P.up[0] + P.fwd[0];
-    double dy = u * P.rgt[1] + v * P.up[1] + P.fwd[1];
-    double dz = u * P.rgt[2] + v * P.up[2] + P.fwd[2];
-    double dd = dx * dx + dy * dy + dz * dz;
-    double inv_n = 1.0 / sqrt(dd);
-    double nx = dx * inv_n, ny = dy * inv_n, nz = dz * inv_n;
-    double x = P.cam[0], y = P.cam[1], z = P.cam[2];
-    double a2 = g.aa;
-    double rho2 = fma(x, x, fma(y, y, z * z));
-    double uu = rho2 - a2;
-    double s_term = sqrt(fma(uu, uu, 4.0 * a2 * y * y));
-    double r2 = 0.5 * (uu + s_term);
-    double r = sqrt(r2);
-    double denom = r2 + a2;
-    double inv_denom = 1.0 / denom;
-    double lx = (r * x - g.a * z) * inv_denom;
-    double ly = y / r;
-    double lz = (r * z + g.a * x) * inv_denom;
-    double denomH = fma(r2, r2, a2 * y * y);
-    double Hs = g.M * r2 * r * (1.0 / denomH);
-    double ldotn = fma(lx, nx, fma(ly, ny, lz * nz));
-    double g_tt = -1.0 + 2.0 * Hs;
-    double g_tn = 2.0 * Hs * ldotn;
-    double g_nn = 1.0 + 2.0 * Hs * ldotn * ldotn;
-    double disc = g_tn * g_tn - g_tt * g_nn;
-    L.kt = (-g_tn - sqrt(disc)) / g_tt;
-    L.pt = g_tt * L.kt + g_tn;
-    double sc = L.kt + ldotn;
-    L.s = V6{x, y, z, nx + 2.0 * Hs * lx * sc, ny + 2.0 * Hs * ly * sc, nz + 2.0 * Hs * lz * sc};
-    L.col[0] = L.col[1] = L.col[2] = 0.0;
-    L.Tr = 1.0;
-    L.h = P.h0;
-    L.have_k1 = false;
-    L.step = 0;
-    L.att_in_step = 0;
-    L.cls = 0;
-    L.tol = step_tolerance(L.s, g, H.tol, H.tol_floor);
-}
-
-// A sample ended: accumulate it, then start the next sample of the pixel
-// or write the pixel (cuda/kernel.cu:132-141) and free the lane.  Samples
-// whose step budget is empty (max_steps = 0 or cutoff >= 1) add nothing.
-__device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const Hot& H) {
-    for (;;) {
-        L.acc[0] = L.acc[0] + L.col[0];
-        L.acc[1] = L.acc[1] + L.col[1];
-        L.acc[2] = L.acc[2] + L.col[2];
-        L.sample += 1;
-        if (L.sample >= H.spp) break;
-        start_sample(L, P, H);
-        if (0 < H.max_steps && L.Tr > H.cutoff) return;
-    }
-    double fs = P.exposure_over_spp;
-    size_t idx = (size_t)L.pix_y * (size_t)P.W + (size_t)L.pix_x;
-    float4 out = make_float4((float)(L.acc[0] * fs), (float)(L.acc[1] * fs), (float)(L.acc[2] * fs), 1.0f);
-    *reinterpret_cast<float4*>(P.hdr + idx * 4) = out;
-    int c = L.cls | (L.overflow ? 16 : 0);
-    if (P.cls) P.cls[idx] = (uint8_t)c;
-    if (L.overflow && P.error_flag) {
-        if (atomicCAS(P.error_flag, 0u, 1u) == 0u) P.error_flag[1] = __float_as_uint((float)L.overflow_T);
-    }
-    L.active = false;
-}
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams* __restrict__ Ps, int n_renders,
@@ -476,8 +154,10 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
                 double py_hit = dense(s.py, ns.py, k1.py, k3.py, k4.py, k5.py, k6.py, k7.py, hs, t);
                 double pz_hit = dense(s.pz, ns.pz, k1.pz, k3.pz, k4.pz, k5.pz, k6.pz, k7.pz, hs, t);
                 if (shade_crossing(P, x_hit, z_hit, px_hit, py_hit, pz_hit, L.pt, L.kt, L.col, L.Tr, L.overflow,
-                                   L.overflow_T))
+                                   L.overflow_T)) {
                     L.cls |= 4;
+                    if (((L.cls >> 5) & 7) < 7) L.cls += 32;  // contributions, saturating at 7
+                }
             }
             if (r_now > H.R_esc) {  // escape: sky along normalize(dx/dl) (kerr_trace.cuh:229-237)
                 bool grid = false;

@@ -785,7 +785,10 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
                         S alpha = 1.0 - exp_(-luma_scaled * 0.5);
                         for (int c = 0; c < 3; ++c) col[c] = col[c] + d_col[c] * alpha * transmittance;
                         transmittance = transmittance * (1.0 - alpha);
-                        if (val(alpha) > 0.0) cls |= KT_FLAG_DISK;
+                        if (val(alpha) > 0.0) {
+                            cls |= KT_FLAG_DISK;
+                            if (((cls >> 5) & 7) < 7) cls += 32;  // contributions, saturating at 7
+                        }
                     }
                 }
             }

@@ -1,9 +1,13 @@
 """Comparison metrics of a GPU frame against the FP64 oracle.
 
 Hit class of a pixel: DISK if any disk emission was composited, else its
-terminal reason (HORIZON, ESCAPE, OTHER).  A boundary pixel is one whose
-oracle 3x3 neighbourhood holds more than one class, or an escape ray on a
-sky grid line (the grid test is a hard threshold on atan2/asin).
+terminal reason (HORIZON, ESCAPE, OTHER).  Its sub-class adds the number
+of disk contributions (bits 5-7 of the class byte).  A boundary pixel is
+one whose oracle 3x3 neighbourhood holds more than one class or
+contribution count, or an escape ray on a sky grid line (the grid test is
+a hard threshold on atan2/asin).  A pixel whose sub-class differs between
+the two frames is a sub-class mismatch; colour is compared only where the
+sub-classes agree.
 Relative error per channel: |d| / max(|ref|, 1e-6 * max|ref|).
 """
 import numpy as np
@@ -18,8 +22,13 @@ def hit_class(cls):
     return out
 
 
+def sub_class(cls):
+    c = np.asarray(cls).astype(np.int32)
+    return hit_class(c) * 8 + ((c >> 5) & 7)
+
+
 def boundary_mask(ref_cls):
-    hc = hit_class(ref_cls)
+    hc = sub_class(ref_cls)
     H, W = hc.shape
     pad = np.pad(hc, 1, mode="edge")
     diff = np.zeros_like(hc, dtype=bool)
@@ -37,7 +46,8 @@ def compare(gpu_hdr, gpu_cls, ref_hdr, ref_cls):
     hg, hr = hit_class(gpu_cls), hit_class(ref_cls)
     bnd = boundary_mask(ref_cls)
     mism = hg != hr
-    same = ~mism
+    sub_mism = sub_class(gpu_cls) != sub_class(ref_cls)
+    same = ~sub_mism
     scale = max(float(np.abs(r).max()), 1e-300)
     den = np.maximum(np.abs(r), 1e-6 * scale)
     rel = np.abs(g - r) / den
@@ -49,6 +59,8 @@ def compare(gpu_hdr, gpu_cls, ref_hdr, ref_cls):
         "class_mismatch": int(mism.sum()),
         "class_mismatch_non_boundary": int((mism & ~bnd).sum()),
         "class_mismatch_frac": float(mism.sum()) / n,
+        "subclass_mismatch": int(sub_mism.sum()),
+        "subclass_mismatch_non_boundary": int((sub_mism & ~bnd).sum()),
         "boundary_frac": float(bnd.sum()) / n,
         "max_rel": float(rel_same.max()) if rel_same.size else 0.0,
         "rms_rel": float(np.sqrt(np.mean(rel_same ** 2))) if rel_same.size else 0.0,
