@@ -61,6 +61,63 @@ def cpu_baseline(cfg, cam, budget_s=20.0):
     }
 
 
+def timed_traces(r, cam, hdr, steps, cls=None):
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for k in range(steps):
+        evs[k][0].record()
+        r.trace(cam, hdr=hdr, cls=cls, with_class=cls is not None)
+        evs[k][1].record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in evs) / steps / 1e3
+
+
+def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
+    """The same frame through the mixed-precision tracer: speed relative to
+    the FP64 tracer and error against the FP64 frame (per-pixel hit class,
+    sub-class, relative HDR error on matching non-boundary pixels)."""
+    import numpy as np
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.renderer import HipRenderer
+    from tests.parity import compare
+    cls64 = torch.empty((cfg.height, cfg.width), dtype=torch.uint8, device="cuda")
+    r64.trace(cam, hdr=hdr64, cls=cls64)
+    torch.cuda.synchronize()
+    ref_hdr, ref_cls = hdr64.cpu().numpy().astype(np.float64), cls64.cpu().numpy()
+    out = {}
+    for tol in (cfg.tolerance, 1e-7):
+        c = cfg.copy()
+        c.precision = K.KT_PRECISION_MIXED
+        c.tolerance = tol
+        rm = HipRenderer(c)
+        hm = rm.alloc_hdr()
+        clm = torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda")
+        rm.trace(cam, hdr=hm, cls=clm)
+        torch.cuda.synchronize()
+        t = timed_traces(rm, cam, hm, args.steps)
+        st = rm.stats()
+        rm.trace(cam, hdr=hm, cls=clm)
+        torch.cuda.synchronize()
+        m = compare(hm.cpu().numpy(), clm.cpu().numpy(), ref_hdr, ref_cls)
+        rays = c.width * c.height * c.spp
+        key = "same_tol" if tol == cfg.tolerance else "tol_1e-7"
+        out[key] = {
+            "tolerance": tol,
+            "trace_ms": round(t * 1e3, 4),
+            "trace_mrays_s": round(rays / t / 1e6, 4),
+            "speedup_vs_fp64": round(t64 / t, 4),
+            "attempts_per_ray": round(st["attempts"] / rays, 3),
+            "algorithmic_tflops": round(trace_flops(st, fc) / t / 1e12, 4),
+            "frac_of_fp32_vector_peak": round(trace_flops(st, fc) / t / PEAK_FP32_VECTOR, 4),
+            "vs_fp64_frame": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac", "class_mismatch_non_boundary",
+                                                 "subclass_mismatch", "max_rel_non_boundary", "rms_rel", "p999_rel")},
+        }
+        rm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +125,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
@@ -150,6 +208,9 @@ def main():
                           "kernels": "bloom_h + post_v"},
     }
     print(f"[bench] timed {args.steps} frames: {ms_per_step:.2f} ms/frame", file=sys.stderr, flush=True)
+    if not args.no_mixed:
+        out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_trace)
+
     if not args.no_cpu:
         print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, cam, args.cpu_budget)

@@ -54,14 +54,13 @@ __device__ __forceinline__ double pow_m02(double x) {
     return ldexp(y * tj, -q);
 }
 
-// cuda/kerr_math.cuh:50-59: Kerr-Schild radius from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0.
+// cuda/kerr_math.cuh:50-59: Kerr-Schild radius from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0
+// (s_sq * rsqrt(s_sq) evaluated as one square root).
 __device__ __forceinline__ double ks_r(double x, double y, double z, double a2) {
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - a2;
-    double s_sq = fma(u, u, 4.0 * a2 * y * y);
-    double inv_s = 1.0 / sqrt(s_sq);
-    double s = s_sq * inv_s;
-    double r2 = 0.5 * (u + s);
+    double s_sq = fma(u, u, 4.0 * (a2 * y) * y);
+    double r2 = 0.5 * (u + sqrt(s_sq));
     return sqrt(r2);
 }
 
@@ -77,78 +76,87 @@ struct Geo {
 
 // cuda/kerr_math.cuh:82-156: Hamiltonian right-hand side in Cartesian
 // Kerr-Schild coordinates, dx/dl = p - 2H (l.p - pt) l and
-// dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p).
-__device__ __forceinline__ V6 rhs(const V6& s, double pt, const Geo& g) {
-    const double x = s.x, y = s.y, z = s.z;
-    const double a2 = g.aa;
+// dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p).  Products that
+// feed a sum are fused with explicit fma; the reciprocals 1/s, 1/r,
+// 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division.
+__device__ __forceinline__ V6 rhs(const V6& st, double pt, const Geo& g) {
+    const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
+    const double a2 = g.aa, a = g.a;
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - a2;
-    double s_sq = fma(u, u, 4.0 * a2 * y * y);
-    double inv_s = 1.0 / sqrt(s_sq);
-    double r2 = 0.5 * (u + s_sq * inv_s);
-    double inv_r = 1.0 / sqrt(r2);
-    double r = r2 * inv_r;
-    double dr2_dx = x * (1.0 + u * inv_s);
+    double ya = a2 * y;
+    double s_sq = fma(u, u, 4.0 * ya * y);
+    double s = sqrt(s_sq);
+    double r2 = 0.5 * (u + s);
+    double r = sqrt(r2);
+    double denom = r2 + a2;
+    double denomH = fma(r2, r2, ya * y);
+    double sr = s * r;
+    double dd = denom * denomH;
+    double q = 1.0 / (sr * dd);
+    double inv_s = (r * dd) * q;
+    double inv_r = (s * dd) * q;
+    double inv_denom = (sr * denomH) * q;
+    double inv_denomH = (sr * denom) * q;
+    double dr2_dx = x * fma(u, inv_s, 1.0);
     // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
     // exact derivative of r^2 is y (1 + (u + 2a^2)/s)
-    double dr2_dy = g.exact ? y * (1.0 + (u + 2.0 * a2) * inv_s) : y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
-    double dr2_dz = z * (1.0 + u * inv_s);
+    double dr2_dy = g.exact ? y * fma(u + 2.0 * a2, inv_s, 1.0) : y * fma(u + 4.0 * a2, 0.5 * inv_s, 1.0);
+    double dr2_dz = z * fma(u, inv_s, 1.0);
     double inv_2r = 0.5 * inv_r;
     double drdx = dr2_dx * inv_2r;
     double drdy = dr2_dy * inv_2r;
     double drdz = dr2_dz * inv_2r;
-    double denom = r2 + a2;
-    double inv_denom = 1.0 / denom;
     double inv_denom2 = inv_denom * inv_denom;
-    double lx = (r * x - g.a * z) * inv_denom;
+    double Nx = fma(r, x, -(a * z));
+    double Nz = fma(r, z, a * x);
+    double lx = Nx * inv_denom;
     double ly = y * inv_r;
-    double lz = (r * z + g.a * x) * inv_denom;
+    double lz = Nz * inv_denom;
     double r3 = r2 * r;
-    double denomH = fma(r2, r2, a2 * y * y);
-    double inv_denomH = 1.0 / denomH;
-    double H = g.M * r3 * inv_denomH;
-    double lp = fma(lx, s.px, fma(ly, s.py, fma(lz, s.pz, -pt)));
+    double H = (g.M * r3) * inv_denomH;
+    double lp = fma(lx, px, fma(ly, py, fma(lz, pz, -pt)));
+    double c = (2.0 * H) * lp;
     V6 d;
-    double c = 2.0 * H * lp;
-    d.x = s.px - c * lx;
-    d.y = s.py - c * ly;
-    d.z = s.pz - c * lz;
-    double dD_dx = 2.0 * r * drdx;
-    double dD_dy = 2.0 * r * drdy;
-    double dD_dz = 2.0 * r * drdz;
-    double Nx = r * x - g.a * z;
-    double Nz = r * z + g.a * x;
-    double dNx_dx = drdx * x + r;
+    d.x = fma(-c, lx, px);
+    d.y = fma(-c, ly, py);
+    d.z = fma(-c, lz, pz);
+    double r_2 = 2.0 * r;
+    double dD_dx = r_2 * drdx;
+    double dD_dy = r_2 * drdy;
+    double dD_dz = r_2 * drdz;
+    double dNx_dx = fma(drdx, x, r);
     double dNx_dy = drdy * x;
-    double dNx_dz = drdz * x - g.a;
-    double dNz_dx = drdx * z + g.a;
+    double dNx_dz = fma(drdz, x, -a);
+    double dNz_dx = fma(drdx, z, a);
     double dNz_dy = drdy * z;
-    double dNz_dz = drdz * z + r;
-    double dlx_dx = (dNx_dx * denom - Nx * dD_dx) * inv_denom2;
-    double dlx_dy = (dNx_dy * denom - Nx * dD_dy) * inv_denom2;
-    double dlx_dz = (dNx_dz * denom - Nx * dD_dz) * inv_denom2;
-    double dlz_dx = (dNz_dx * denom - Nz * dD_dx) * inv_denom2;
-    double dlz_dy = (dNz_dy * denom - Nz * dD_dy) * inv_denom2;
-    double dlz_dz = (dNz_dz * denom - Nz * dD_dz) * inv_denom2;
+    double dNz_dz = fma(drdz, z, r);
+    double dlx_dx = fma(dNx_dx, denom, -(Nx * dD_dx)) * inv_denom2;
+    double dlx_dy = fma(dNx_dy, denom, -(Nx * dD_dy)) * inv_denom2;
+    double dlx_dz = fma(dNx_dz, denom, -(Nx * dD_dz)) * inv_denom2;
+    double dlz_dx = fma(dNz_dx, denom, -(Nz * dD_dx)) * inv_denom2;
+    double dlz_dy = fma(dNz_dy, denom, -(Nz * dD_dy)) * inv_denom2;
+    double dlz_dz = fma(dNz_dz, denom, -(Nz * dD_dz)) * inv_denom2;
     double inv_r2 = inv_r * inv_r;
-    double dly_dx = -y * drdx * inv_r2;
-    double dly_dy = (r - y * drdy) * inv_r2;
-    double dly_dz = -y * drdz * inv_r2;
-    double dDenH_dx = 4.0 * r3 * drdx;
-    double dDenH_dy = fma(4.0 * r3, drdy, 2.0 * a2 * y);
-    double dDenH_dz = 4.0 * r3 * drdz;
-    double inv_denomH2 = inv_denomH * inv_denomH;
-    double factor_H = g.M * inv_denomH2;
-    double dH_dx = factor_H * (3.0 * r2 * drdx * denomH - r3 * dDenH_dx);
-    double dH_dy = factor_H * (3.0 * r2 * drdy * denomH - r3 * dDenH_dy);
-    double dH_dz = factor_H * (3.0 * r2 * drdz * denomH - r3 * dDenH_dz);
-    double dlp_dx = dlx_dx * s.px + dly_dx * s.py + dlz_dx * s.pz;
-    double dlp_dy = dlx_dy * s.px + dly_dy * s.py + dlz_dy * s.pz;
-    double dlp_dz = dlx_dz * s.px + dly_dz * s.py + dlz_dz * s.pz;
+    double dly_dx = -(y * drdx) * inv_r2;
+    double dly_dy = fma(-y, drdy, r) * inv_r2;
+    double dly_dz = -(y * drdz) * inv_r2;
+    double r3x4 = 4.0 * r3;
+    double dDenH_dx = r3x4 * drdx;
+    double dDenH_dy = fma(r3x4, drdy, 2.0 * ya);
+    double dDenH_dz = r3x4 * drdz;
+    double factor_H = g.M * (inv_denomH * inv_denomH);
+    double t3 = (3.0 * r2) * denomH;
+    double dH_dx = factor_H * fma(t3, drdx, -(r3 * dDenH_dx));
+    double dH_dy = factor_H * fma(t3, drdy, -(r3 * dDenH_dy));
+    double dH_dz = factor_H * fma(t3, drdz, -(r3 * dDenH_dz));
+    double dlp_dx = fma(dlx_dx, px, fma(dly_dx, py, dlz_dx * pz));
+    double dlp_dy = fma(dlx_dy, px, fma(dly_dy, py, dlz_dy * pz));
+    double dlp_dz = fma(dlx_dz, px, fma(dly_dz, py, dlz_dz * pz));
     double lp2 = lp * lp;
-    d.px = dH_dx * lp2 + c * dlp_dx;
-    d.py = dH_dy * lp2 + c * dlp_dy;
-    d.pz = dH_dz * lp2 + c * dlp_dz;
+    d.px = fma(dH_dx, lp2, c * dlp_dx);
+    d.py = fma(dH_dy, lp2, c * dlp_dy);
+    d.pz = fma(dH_dz, lp2, c * dlp_dz);
     return d;
 }
 

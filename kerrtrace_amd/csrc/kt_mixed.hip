@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void trace_mixed_persistent(const RenderParams
         if (!done) {
             L.step += 1;
             if (L.step < H.max_steps && L.Tr > H.cutoff) {
-                L.tol = step_tolerance(L.s, H.g, H.tol, H.tol_floor);
+                L.tol = step_tolerance(L.s, r_now, H.g, H.tol, H.tol_floor);
             } else {
                 done = true;
             }

@@ -88,34 +88,39 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
         const V6 k1 = L.k1;
         const double pt = L.pt;
         const double hs = L.h;
-        V6 k2 = rhs(V6{s.x + hs * (KT_A21 * k1.x), s.y + hs * (KT_A21 * k1.y), s.z + hs * (KT_A21 * k1.z),
-                       s.px + hs * (KT_A21 * k1.px), s.py + hs * (KT_A21 * k1.py), s.pz + hs * (KT_A21 * k1.pz)},
-                    pt, g);
-#define KT_ST3(c) (s.c + hs * (KT_A31 * k1.c + KT_A32 * k2.c))
-        V6 k3 = rhs(V6{KT_ST3(x), KT_ST3(y), KT_ST3(z), KT_ST3(px), KT_ST3(py), KT_ST3(pz)}, pt, g);
-#define KT_ST4(c) (s.c + hs * (KT_A41 * k1.c + KT_A42 * k2.c + KT_A43 * k3.c))
-        V6 k4 = rhs(V6{KT_ST4(x), KT_ST4(y), KT_ST4(z), KT_ST4(px), KT_ST4(py), KT_ST4(pz)}, pt, g);
-#define KT_ST5(c) (s.c + hs * (KT_A51 * k1.c + KT_A52 * k2.c + KT_A53 * k3.c + KT_A54 * k4.c))
-        V6 k5 = rhs(V6{KT_ST5(x), KT_ST5(y), KT_ST5(z), KT_ST5(px), KT_ST5(py), KT_ST5(pz)}, pt, g);
-#define KT_ST6(c) (s.c + hs * (KT_A61 * k1.c + KT_A62 * k2.c + KT_A63 * k3.c + KT_A64 * k4.c + KT_A65 * k5.c))
-        V6 k6 = rhs(V6{KT_ST6(x), KT_ST6(y), KT_ST6(z), KT_ST6(px), KT_ST6(py), KT_ST6(pz)}, pt, g);
+        // stage inputs s + h * sum_j a_ij k_j as fma chains
+#define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
+#define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
+#define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
+#define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
+#define KT_ST6(c) \
+    fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
+#define KT_ST7(c) \
+    fma(hs, fma(KT_B6, k6.c, fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))), s.c)
+#define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
+        V6 k2 = rhs(KT_STAGE(KT_ST2), pt, g);
+        V6 k3 = rhs(KT_STAGE(KT_ST3), pt, g);
+        V6 k4 = rhs(KT_STAGE(KT_ST4), pt, g);
+        V6 k5 = rhs(KT_STAGE(KT_ST5), pt, g);
+        V6 k6 = rhs(KT_STAGE(KT_ST6), pt, g);
         // the 7th stage input is the 5th-order solution (a7j = b_j)
-#define KT_ST7(c) (s.c + hs * (KT_B1 * k1.c + KT_B3 * k3.c + KT_B4 * k4.c + KT_B5 * k5.c + KT_B6 * k6.c))
-        const V6 ns{KT_ST7(x), KT_ST7(y), KT_ST7(z), KT_ST7(px), KT_ST7(py), KT_ST7(pz)};
+        const V6 ns = KT_STAGE(KT_ST7);
         V6 k7 = rhs(ns, pt, g);
         cnt.rhs += 6;
         // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
         const double tol = L.tol;
-#define KT_ERR(c)                                                                                            \
-    (fabs(L.h * (KT_E1 * k1.c + KT_E3 * k3.c + KT_E4 * k4.c + KT_E5 * k5.c + KT_E6 * k6.c + KT_E7 * k7.c)) / \
+#define KT_ERR(c)                                                                                               \
+    (fabs(hs * fma(KT_E7, k7.c, fma(KT_E6, k6.c, fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))))) / \
      (tol * fmax(fabs(s.c), 1.0)))
         double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
         double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
+#undef KT_ST2
 #undef KT_ST3
 #undef KT_ST4
 #undef KT_ST5
 #undef KT_ST6
 #undef KT_ST7
+#undef KT_STAGE
 #undef KT_ERR
         double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
         cnt.attempts++;
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
         if (!done) {
             L.step += 1;
             if (L.step < H.max_steps && L.Tr > H.cutoff) {
-                L.tol = step_tolerance(L.s, g, H.tol, H.tol_floor);
+                L.tol = step_tolerance(L.s, r_now, g, H.tol, H.tol_floor);
             } else {
                 done = true;
             }

@@ -41,10 +41,12 @@ namespace kt {
 
 static constexpr int kWave = 64;
 
-// Potential-scaled tolerance of the step that starts at s
-// (cuda/kerr_trace.cuh:67-73, with the tolerance floor extension).
-__device__ __forceinline__ double step_tolerance(const V6& s, const Geo& g, double tol, double tol_floor) {
-    double r_c = ks_r(s.x, s.y, s.z, g.aa);
+// Potential-scaled tolerance of the step that starts at s, whose
+// Kerr-Schild radius is r_c (cuda/kerr_trace.cuh:67-73, with the tolerance
+// floor extension).  The radius is the one the previous commit computed
+// for its horizon and escape tests, so it is not solved again.
+__device__ __forceinline__ double step_tolerance(const V6& s, double r_c, const Geo& g, double tol,
+                                                 double tol_floor) {
     double r2_c = r_c * r_c;
     double dH = fma(r2_c, r2_c, g.aa * s.y * s.y);
     double Hc = g.M * r_c * r2_c * (1.0 / dH);
@@ -306,7 +308,7 @@ __device__ __forceinline__ void start_sample(Lane& L, const RenderParams& P, con
     L.step = 0;
     L.att_in_step = 0;
     L.cls = 0;
-    L.tol = step_tolerance(L.s, g, H.tol, H.tol_floor);
+    L.tol = step_tolerance(L.s, ks_r(L.s.x, L.s.y, L.s.z, g.aa), g, H.tol, H.tol_floor);
 }
 
 // A sample ended: accumulate it, then start the next sample of the pixel

@@ -259,15 +259,14 @@ static int derive(const kt_config& c, S spin_s, S mdot_s, Params<S>& p, const ch
 // ---------------------------------------------------------------- geometry
 template <class S> struct State { S x, y, z, px, py, pz; };
 
-// cuda/kerr_math.cuh:50-59
+// cuda/kerr_math.cuh:50-59.  s_sq * rsqrt(s_sq) is sqrt(s_sq); it is
+// evaluated as one square root.
 template <class S>
 S ks_r(S x, S y, S z, S a2) {
     S rho2 = fma_(x, x, fma_(y, y, z * z));
     S u = rho2 - a2;
-    S s_sq = fma_(u, u, 4.0 * a2 * y * y);
-    S inv_s = 1.0 / sqrt_(s_sq);
-    S s = s_sq * inv_s;
-    S r2 = 0.5 * (u + s);
+    S s_sq = fma_(u, u, 4.0 * (a2 * y) * y);
+    S r2 = 0.5 * (u + sqrt_(s_sq));
     return sqrt_(r2);
 }
 
@@ -298,78 +297,89 @@ void ks_bl(S x, S y, S z, const Params<S>& p, S& r, S& sin_th, S& cos_th, S& phi
 // cuda/kerr_math.cuh:82-156 -- Hamiltonian right-hand side in Cartesian
 // Kerr-Schild coordinates: dx/dl = p - 2H(l.p - pt) l,
 // dp/dl = dH/dx (l.p - pt)^2 + 2H (l.p - pt) d(l.p)/dx.
+// Evaluation order (the device tracer repeats it operation for operation):
+// products that feed a sum are fused with explicit fma_, and the four
+// reciprocals 1/s, 1/r, 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division
+// q = 1/(s r (r^2+a^2)(r^4+a^2y^2)).
 template <class S>
-State<S> get_derivs(const State<S>& s, S pt, const Params<S>& p) {
-    S x = s.x, y = s.y, z = s.z;
-    S a2 = p.aa;
+State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
+    const S x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
+    const S a2 = p.aa, a = p.a;
     S rho2 = fma_(x, x, fma_(y, y, z * z));
     S u = rho2 - a2;
-    S s_sq = fma_(u, u, 4.0 * a2 * y * y);
-    S inv_s = 1.0 / sqrt_(s_sq);
-    S r2 = 0.5 * (u + s_sq * inv_s);
-    S inv_r = 1.0 / sqrt_(r2);
-    S r = r2 * inv_r;
-    S dr2_dx = x * (1.0 + u * inv_s);
+    S ya = a2 * y;
+    S s_sq = fma_(u, u, 4.0 * ya * y);
+    S s = sqrt_(s_sq);
+    S r2 = 0.5 * (u + s);
+    S r = sqrt_(r2);
+    S denom = r2 + a2;
+    S denomH = fma_(r2, r2, ya * y);
+    S sr = s * r;
+    S dd = denom * denomH;
+    S q = 1.0 / (sr * dd);
+    S inv_s = (r * dd) * q;
+    S inv_r = (s * dd) * q;
+    S inv_denom = (sr * denomH) * q;
+    S inv_denomH = (sr * denom) * q;
+    S dr2_dx = x * fma_(u, inv_s, S(1.0));
     // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
     // Hamiltonian mode uses the exact derivative y (1 + (u + 2a^2)/s)
-    S dr2_dy = p.rhs_exact ? y * (1.0 + (u + 2.0 * a2) * inv_s) : y * (1.0 + (u + 4.0 * a2) * (0.5 * inv_s));
-    S dr2_dz = z * (1.0 + u * inv_s);
+    S dr2_dy = p.rhs_exact ? y * fma_(u + 2.0 * a2, inv_s, S(1.0)) : y * fma_(u + 4.0 * a2, 0.5 * inv_s, S(1.0));
+    S dr2_dz = z * fma_(u, inv_s, S(1.0));
     S inv_2r = 0.5 * inv_r;
     S drdx = dr2_dx * inv_2r;
     S drdy = dr2_dy * inv_2r;
     S drdz = dr2_dz * inv_2r;
-    S denom = r2 + a2;
-    S inv_denom = 1.0 / denom;
     S inv_denom2 = inv_denom * inv_denom;
-    S lx = (r * x - p.a * z) * inv_denom;
+    S Nx = fma_(r, x, -(a * z));
+    S Nz = fma_(r, z, a * x);
+    S lx = Nx * inv_denom;
     S ly = y * inv_r;
-    S lz = (r * z + p.a * x) * inv_denom;
+    S lz = Nz * inv_denom;
     S r3 = r2 * r;
-    S denomH = fma_(r2, r2, a2 * y * y);
-    S inv_denomH = 1.0 / denomH;
-    S H = p.M * r3 * inv_denomH;
-    S lp = fma_(lx, s.px, fma_(ly, s.py, fma_(lz, s.pz, -pt)));
+    S H = (p.M * r3) * inv_denomH;
+    S lp = fma_(lx, px, fma_(ly, py, fma_(lz, pz, -pt)));
+    S c = (2.0 * H) * lp;
     State<S> d;
-    S common_H_lp = 2.0 * H * lp;
-    d.x = s.px - common_H_lp * lx;
-    d.y = s.py - common_H_lp * ly;
-    d.z = s.pz - common_H_lp * lz;
-    S dD_dx = 2.0 * r * drdx;
-    S dD_dy = 2.0 * r * drdy;
-    S dD_dz = 2.0 * r * drdz;
-    S Nx = r * x - p.a * z;
-    S Nz = r * z + p.a * x;
-    S dNx_dx = drdx * x + r;
+    d.x = fma_(-c, lx, px);
+    d.y = fma_(-c, ly, py);
+    d.z = fma_(-c, lz, pz);
+    S r_2 = 2.0 * r;
+    S dD_dx = r_2 * drdx;
+    S dD_dy = r_2 * drdy;
+    S dD_dz = r_2 * drdz;
+    S dNx_dx = fma_(drdx, x, r);
     S dNx_dy = drdy * x;
-    S dNx_dz = drdz * x - p.a;
-    S dNz_dx = drdx * z + p.a;
+    S dNx_dz = fma_(drdz, x, -a);
+    S dNz_dx = fma_(drdx, z, a);
     S dNz_dy = drdy * z;
-    S dNz_dz = drdz * z + r;
-    S dlx_dx = (dNx_dx * denom - Nx * dD_dx) * inv_denom2;
-    S dlx_dy = (dNx_dy * denom - Nx * dD_dy) * inv_denom2;
-    S dlx_dz = (dNx_dz * denom - Nx * dD_dz) * inv_denom2;
-    S dlz_dx = (dNz_dx * denom - Nz * dD_dx) * inv_denom2;
-    S dlz_dy = (dNz_dy * denom - Nz * dD_dy) * inv_denom2;
-    S dlz_dz = (dNz_dz * denom - Nz * dD_dz) * inv_denom2;
+    S dNz_dz = fma_(drdz, z, r);
+    S dlx_dx = fma_(dNx_dx, denom, -(Nx * dD_dx)) * inv_denom2;
+    S dlx_dy = fma_(dNx_dy, denom, -(Nx * dD_dy)) * inv_denom2;
+    S dlx_dz = fma_(dNx_dz, denom, -(Nx * dD_dz)) * inv_denom2;
+    S dlz_dx = fma_(dNz_dx, denom, -(Nz * dD_dx)) * inv_denom2;
+    S dlz_dy = fma_(dNz_dy, denom, -(Nz * dD_dy)) * inv_denom2;
+    S dlz_dz = fma_(dNz_dz, denom, -(Nz * dD_dz)) * inv_denom2;
     S inv_r2 = inv_r * inv_r;
-    S dly_dx = -y * drdx * inv_r2;
-    S dly_dy = (r - y * drdy) * inv_r2;
-    S dly_dz = -y * drdz * inv_r2;
-    S dDenH_dx = 4.0 * r3 * drdx;
-    S dDenH_dy = fma_(4.0 * r3, drdy, 2.0 * a2 * y);
-    S dDenH_dz = 4.0 * r3 * drdz;
-    S inv_denomH2 = inv_denomH * inv_denomH;
-    S factor_H = p.M * inv_denomH2;
-    S dH_dx = factor_H * (3.0 * r2 * drdx * denomH - r3 * dDenH_dx);
-    S dH_dy = factor_H * (3.0 * r2 * drdy * denomH - r3 * dDenH_dy);
-    S dH_dz = factor_H * (3.0 * r2 * drdz * denomH - r3 * dDenH_dz);
-    S dlp_dx = dlx_dx * s.px + dly_dx * s.py + dlz_dx * s.pz;
-    S dlp_dy = dlx_dy * s.px + dly_dy * s.py + dlz_dy * s.pz;
-    S dlp_dz = dlx_dz * s.px + dly_dz * s.py + dlz_dz * s.pz;
+    S dly_dx = -(y * drdx) * inv_r2;
+    S dly_dy = fma_(-y, drdy, r) * inv_r2;
+    S dly_dz = -(y * drdz) * inv_r2;
+    S r3x4 = 4.0 * r3;
+    S dDenH_dx = r3x4 * drdx;
+    S dDenH_dy = fma_(r3x4, drdy, 2.0 * ya);
+    S dDenH_dz = r3x4 * drdz;
+    S factor_H = p.M * (inv_denomH * inv_denomH);
+    S t3 = (3.0 * r2) * denomH;
+    S dH_dx = factor_H * fma_(t3, drdx, -(r3 * dDenH_dx));
+    S dH_dy = factor_H * fma_(t3, drdy, -(r3 * dDenH_dy));
+    S dH_dz = factor_H * fma_(t3, drdz, -(r3 * dDenH_dz));
+    S dlp_dx = fma_(dlx_dx, px, fma_(dly_dx, py, dlz_dx * pz));
+    S dlp_dy = fma_(dlx_dy, px, fma_(dly_dy, py, dlz_dy * pz));
+    S dlp_dz = fma_(dlx_dz, px, fma_(dly_dz, py, dlz_dz * pz));
     S lp2 = lp * lp;
-    d.px = dH_dx * lp2 + common_H_lp * dlp_dx;
-    d.py = dH_dy * lp2 + common_H_lp * dlp_dy;
-    d.pz = dH_dz * lp2 + common_H_lp * dlp_dz;
+    d.px = fma_(dH_dx, lp2, c * dlp_dx);
+    d.py = fma_(dH_dy, lp2, c * dlp_dy);
+    d.pz = fma_(dH_dz, lp2, c * dlp_dz);
     return d;
 }
 
@@ -577,13 +587,15 @@ void dp_attempt(const State<S>& s, S pt, S hs, double tol, const State<S>& k1, c
     A.k1 = k1;
     State<S>& k2 = A.k2; State<S>& k3 = A.k3; State<S>& k4 = A.k4; State<S>& k5 = A.k5;
     State<S>& k6 = A.k6; State<S>& k7 = A.k7;
-#define KTO_STAGE(E) State<S>{s.x + hs * (E(x)), s.y + hs * (E(y)), s.z + hs * (E(z)), s.px + hs * (E(px)), s.py + hs * (E(py)), s.pz + hs * (E(pz))}
-#define E2(c) (T::a21 * k1.c)
-#define E3(c) (T::a31 * k1.c + T::a32 * k2.c)
-#define E4(c) (T::a41 * k1.c + T::a42 * k2.c + T::a43 * k3.c)
-#define E5(c) (T::a51 * k1.c + T::a52 * k2.c + T::a53 * k3.c + T::a54 * k4.c)
-#define E6(c) (T::a61 * k1.c + T::a62 * k2.c + T::a63 * k3.c + T::a64 * k4.c + T::a65 * k5.c)
-#define E7(c) (T::b1 * k1.c + T::b3 * k3.c + T::b4 * k4.c + T::b5 * k5.c + T::b6 * k6.c)
+    // stage input s + h * sum_j a_ij k_j as an fma chain (first term a
+    // product, then fma per further term, then one fma with h)
+#define KTO_STAGE(E) State<S>{E(x), E(y), E(z), E(px), E(py), E(pz)}
+#define E2(c) fma_(hs, T::a21 * k1.c, s.c)
+#define E3(c) fma_(hs, fma_(S(T::a32), k2.c, T::a31 * k1.c), s.c)
+#define E4(c) fma_(hs, fma_(S(T::a43), k3.c, fma_(S(T::a42), k2.c, T::a41 * k1.c)), s.c)
+#define E5(c) fma_(hs, fma_(S(T::a54), k4.c, fma_(S(T::a53), k3.c, fma_(S(T::a52), k2.c, T::a51 * k1.c))), s.c)
+#define E6(c) fma_(hs, fma_(S(T::a65), k5.c, fma_(S(T::a64), k4.c, fma_(S(T::a63), k3.c, fma_(S(T::a62), k2.c, T::a61 * k1.c)))), s.c)
+#define E7(c) fma_(hs, fma_(S(T::b6), k6.c, fma_(S(T::b5), k5.c, fma_(S(T::b4), k4.c, fma_(S(T::b3), k3.c, T::b1 * k1.c)))), s.c)
     k2 = get_derivs(KTO_STAGE(E2), pt, p);
     k3 = get_derivs(KTO_STAGE(E3), pt, p);
     k4 = get_derivs(KTO_STAGE(E4), pt, p);
@@ -598,7 +610,8 @@ void dp_attempt(const State<S>& s, S pt, S hs, double tol, const State<S>& k1, c
 #undef E5
 #undef E6
 #undef E7
-#define KTO_ERR(c) (fabs_(hs * (T::dc1 * k1.c + T::dc3 * k3.c + T::dc4 * k4.c + T::dc5 * k5.c + T::dc6 * k6.c + T::dc7 * k7.c)) / (tol * fmax_(fabs_(s.c), S(1.0))))
+    // embedded error (b - bhat) . k, max norm relative to tol * max(|s_i|, 1)
+#define KTO_ERR(c) (fabs_(hs * fma_(S(T::dc7), k7.c, fma_(S(T::dc6), k6.c, fma_(S(T::dc5), k5.c, fma_(S(T::dc4), k4.c, fma_(S(T::dc3), k3.c, T::dc1 * k1.c)))))) / (tol * fmax_(fabs_(s.c), S(1.0))))
     S ex = KTO_ERR(x), ey = KTO_ERR(y), ez = KTO_ERR(z);
     S epx = KTO_ERR(px), epy = KTO_ERR(py), epz = KTO_ERR(pz);
 #undef KTO_ERR
@@ -1171,7 +1184,8 @@ int kto_flop_counts(const kt_config* cfg, const double s6[6], double pt, uint64_
     Fl hn = hs * 0.9 * pow_m02(Fl(0.5));
     (void)hn;
     out[1] = op_counter().flops;
-    // per-step commit: ks_r for the tolerance, H, tq, tol, ks_r after commit
+    // per-step commit: the radius after the step (horizon/escape tests, and
+    // reused for the next step's tolerance), then H, tq and the tolerance
     op_counter() = OpCount();
     Fl r_c = ks_r(s.x, s.y, s.z, p.aa);
     Fl r2c = r_c * r_c;
@@ -1180,8 +1194,6 @@ int kto_flop_counts(const kt_config* cfg, const double s6[6], double pt, uint64_
     Fl tq = (Hc - 0.05) / (0.3 - 0.05);
     Fl tolv = Fl(5.0) - 5.0 * tq;
     (void)tolv;
-    Fl r_now = ks_r(s.x, s.y, s.z, p.aa);
-    (void)r_now;
     out[2] = op_counter().flops;
     out[3] = 0;
     return KT_OK;

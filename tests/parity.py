@@ -1,11 +1,12 @@
 """Comparison metrics of a GPU frame against the FP64 oracle.
 
 Hit class of a pixel: DISK if any disk emission was composited, else its
-terminal reason (HORIZON, ESCAPE, OTHER).  Its sub-class adds the number
-of disk contributions (bits 5-7 of the class byte).  A boundary pixel is
+terminal reason (HORIZON, ESCAPE, OTHER).  Its sub-class adds the sky-grid
+flag and the number of disk contributions (bits 3 and 5-7 of the class
+byte).  A boundary pixel is
 one whose oracle 3x3 neighbourhood holds more than one class or
-contribution count, or an escape ray on a sky grid line (the grid test is
-a hard threshold on atan2/asin).  A pixel whose sub-class differs between
+contribution count, or an escape ray on a sky grid line in either frame
+(the grid test is a hard threshold on atan2/asin).  A pixel whose sub-class differs between
 the two frames is a sub-class mismatch; colour is compared only where the
 sub-classes agree.
 Relative error per channel: |d| / max(|ref|, 1e-6 * max|ref|).
@@ -24,7 +25,7 @@ def hit_class(cls):
 
 def sub_class(cls):
     c = np.asarray(cls).astype(np.int32)
-    return hit_class(c) * 8 + ((c >> 5) & 7)
+    return hit_class(c) * 16 + (c & 8) + ((c >> 5) & 7)
 
 
 def boundary_mask(ref_cls):
@@ -44,7 +45,7 @@ def compare(gpu_hdr, gpu_cls, ref_hdr, ref_cls):
     g = np.asarray(gpu_hdr, dtype=np.float64)[..., :3]
     r = np.asarray(ref_hdr, dtype=np.float64)[..., :3]
     hg, hr = hit_class(gpu_cls), hit_class(ref_cls)
-    bnd = boundary_mask(ref_cls)
+    bnd = boundary_mask(ref_cls) | ((np.asarray(gpu_cls).astype(np.int32) & 8) != 0)
     mism = hg != hr
     sub_mism = sub_class(gpu_cls) != sub_class(ref_cls)
     same = ~sub_mism

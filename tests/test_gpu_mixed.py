@@ -49,10 +49,11 @@ def test_mixed_matches_fp64_oracle(torch_gpu, name, size, tol):
     say(f"[mixed] {name}: {m}; attempts gpu {st['attempts']} oracle {ost.attempts}")
     assert m["class_mismatch_non_boundary"] <= 2
     assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
+    assert m["subclass_mismatch"] <= MIXED_MAX_MISMATCH_FRAC * m["pixels"]
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
     # the split stage form keeps FP32 noise out of the error norm: the step
-    # count stays within a few percent of the FP64 integration
-    assert st["attempts"] <= 1.05 * ost.attempts
+    # count stays close to the FP64 integration's
+    assert st["attempts"] <= 1.15 * ost.attempts
 
 
 @pytest.mark.parametrize("spin", [0.0, 0.5, 0.9, 0.998])
@@ -67,4 +68,5 @@ def test_mixed_spins_inclination_60(torch_gpu, spin):
     m = compare(g_hdr, g_cls, o_hdr, o_cls)
     say(f"[mixed] spin {spin}: {m}")
     assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
+    assert m["subclass_mismatch"] <= MIXED_MAX_MISMATCH_FRAC * m["pixels"]
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
