@@ -29,12 +29,15 @@
 namespace kt {
 
 template <bool BATCH>
-__global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams* __restrict__ Ps, int n_renders,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_fp64_persistent(const RenderParams* __restrict__ Ps, int n_renders,
                                                              uint64_t total_items, int refill_threshold,
                                                              DevCounters* __restrict__ counters) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
     Lane L;
     L.active = false;
+    L.finished = false;
+    L.post = false;
+    L.crossing = false;
     L.rid = 0;
     L.overflow = false;
     L.overflow_T = 0.0;
@@ -44,38 +47,22 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
     Hot H = load_hot(prm<BATCH>(Ps, 0));
 
     for (;;) {
-        // ---- refill idle lanes from the work queue (wave-uniform branch)
-        uint64_t idle = __ballot(!L.active);
-        if (idle != 0 && !drained) {
-            int n_idle = __popcll(idle);
-            if (n_idle >= refill_threshold || idle == __ballot(1)) {
-                int leader = __ffsll((unsigned long long)idle) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&counters->queue_head, (unsigned long long)n_idle);
-                base = __shfl(base, leader);
-                if (base + (unsigned long long)n_idle >= total_items) drained = true;
-                if (!L.active) {
-                    uint64_t item = base + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
-                    int rid = 0, px = 0, py = 0;
-                    if (item < total_items && decode_item<BATCH>(Ps, n_renders, item, rid, px, py)) {
-                        L.rid = rid;
-                        L.pix_x = px;
-                        L.pix_y = py;
-                        const RenderParams& P = prm<BATCH>(Ps, rid);
-                        if (BATCH) H = load_hot(P);
-                        L.sample = 0;
-                        L.acc[0] = L.acc[1] = L.acc[2] = 0.0;
-                        L.overflow = false;
-                        L.ray_att = 0;
-                        L.active = true;
-                        start_sample(L, P, H);
-                        if (!(0 < H.max_steps && L.Tr > H.cutoff)) end_sample(L, P, H);
-                    }
-                }
-            }
+        // ---- second half of last iteration's commit, then rays that ended:
+        // write the pixel or start its next sample.  Both run here, where no
+        // stage derivative is live.
+        if (L.post) {
+            L.post = false;
+            post_commit(L, prm<BATCH>(Ps, L.rid), H, cnt);
         }
+        if (L.finished) {
+            L.finished = false;
+            cnt.max_att = max(cnt.max_att, L.ray_att);
+            end_sample(L, prm<BATCH>(Ps, L.rid), H);
+        }
+        // ---- refill idle lanes from the work queue (wave-uniform branch)
+        refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
         if (drained && __ballot(L.active) == 0) break;
-        if (!L.active) continue;
+        if (!L.active || L.finished) continue;
 
         // ---- one DP5(4) attempt (cuda/kerr_trace.cuh:83-122)
         const Geo g = H.g;
@@ -85,43 +72,55 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
             L.have_k1 = true;
             cnt.rhs++;
         }
-        const V6 k1 = L.k1;
         const double pt = L.pt;
         const double hs = L.h;
-        // stage inputs s + h * sum_j a_ij k_j as fma chains
+        V6 ns, eacc;
+        {
+            const V6 k1 = L.k1;
+            // stage inputs s + h * sum_j a_ij k_j as fma chains
 #define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
 #define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
 #define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
 #define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
 #define KT_ST6(c) \
     fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
-#define KT_ST7(c) \
-    fma(hs, fma(KT_B6, k6.c, fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))), s.c)
 #define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
-        V6 k2 = rhs(KT_STAGE(KT_ST2), pt, g);
-        V6 k3 = rhs(KT_STAGE(KT_ST3), pt, g);
-        V6 k4 = rhs(KT_STAGE(KT_ST4), pt, g);
-        V6 k5 = rhs(KT_STAGE(KT_ST5), pt, g);
-        V6 k6 = rhs(KT_STAGE(KT_ST6), pt, g);
-        // the 7th stage input is the 5th-order solution (a7j = b_j)
-        const V6 ns = KT_STAGE(KT_ST7);
-        V6 k7 = rhs(ns, pt, g);
-        cnt.rhs += 6;
-        // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
-        const double tol = L.tol;
-#define KT_ERR(c)                                                                                               \
-    (fabs(hs * fma(KT_E7, k7.c, fma(KT_E6, k6.c, fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))))) / \
-     (tol * fmax(fabs(s.c), 1.0)))
-        double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
-        double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
+            const V6 k2 = rhs(KT_STAGE(KT_ST2), pt, g);
+            const V6 k3 = rhs(KT_STAGE(KT_ST3), pt, g);
+            const V6 k4 = rhs(KT_STAGE(KT_ST4), pt, g);
+            const V6 k5 = rhs(KT_STAGE(KT_ST5), pt, g);
+            const V6 in6 = KT_STAGE(KT_ST6);
+            // The 5th-order solution b.k and the error (b - bhat).k are fma
+            // chains in stage order; their leading terms are folded now so
+            // k2..k5 are dead while stage 6 and 7 are evaluated.
+#define KT_B4ACC(c) fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))
+#define KT_E5ACC(c) fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))
+            V6 bacc = KT_STAGE(KT_B4ACC);
+            eacc = KT_STAGE(KT_E5ACC);
+            const V6 k6 = rhs(in6, pt, g);
+#define KT_NS(c) fma(hs, fma(KT_B6, k6.c, bacc.c), s.c)
+#define KT_EACC6(c) fma(KT_E6, k6.c, eacc.c)
+            ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
+            eacc = KT_STAGE(KT_EACC6);
 #undef KT_ST2
 #undef KT_ST3
 #undef KT_ST4
 #undef KT_ST5
 #undef KT_ST6
-#undef KT_ST7
-#undef KT_STAGE
+#undef KT_B4ACC
+#undef KT_E5ACC
+#undef KT_NS
+#undef KT_EACC6
+        }
+        const V6 k7 = rhs(ns, pt, g);
+        cnt.rhs += 6;
+        // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
+        const double tol = L.tol;
+#define KT_ERR(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
+        double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
+        double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
 #undef KT_ERR
+#undef KT_STAGE
         double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
         cnt.attempts++;
         L.ray_att++;
@@ -133,65 +132,9 @@ __global__ __launch_bounds__(256) void trace_fp64_persistent(const RenderParams*
 
         // ---- commit; a rejected last attempt is committed too (kerr_trace.cuh:123-134)
         if (accepted) cnt.accepted++;
-        L.s = ns;
-        L.att_in_step = 0;
-        double r_now = ks_r(ns.x, ns.y, ns.z, g.aa);
-        bool done = false;
-        const RenderParams& P = prm<BATCH>(Ps, L.rid);
-        if (r_now < H.rh_eps) {  // horizon, tested before the disk (kerr_trace.cuh:135-137)
-            L.cls = (L.cls & ~3) | 1;
-            done = true;
-        } else {
-            if ((s.y < 0.0 && ns.y >= 0.0) || (s.y > 0.0 && ns.y <= 0.0)) {
-                // equatorial crossing on the dense output (kerr_trace.cuh:138-174)
-                cnt.crossings++;
-                double t;
-                if (P.disk_root_mode == 1) {
-                    t = disk_root(s.y, ns.y, k1.y, k3.y, k4.y, k5.y, k6.y, k7.y, hs);
-                } else {
-                    double den = s.y - ns.y;
-                    t = den != 0.0 ? s.y / den : 0.5;
-                    t = fmin(fmax(t, 0.0), 1.0);
-                }
-                double x_hit = dense(s.x, ns.x, k1.x, k3.x, k4.x, k5.x, k6.x, k7.x, hs, t);
-                double z_hit = dense(s.z, ns.z, k1.z, k3.z, k4.z, k5.z, k6.z, k7.z, hs, t);
-                double px_hit = dense(s.px, ns.px, k1.px, k3.px, k4.px, k5.px, k6.px, k7.px, hs, t);
-                double py_hit = dense(s.py, ns.py, k1.py, k3.py, k4.py, k5.py, k6.py, k7.py, hs, t);
-                double pz_hit = dense(s.pz, ns.pz, k1.pz, k3.pz, k4.pz, k5.pz, k6.pz, k7.pz, hs, t);
-                if (shade_crossing(P, x_hit, z_hit, px_hit, py_hit, pz_hit, L.pt, L.kt, L.col, L.Tr, L.overflow,
-                                   L.overflow_T)) {
-                    L.cls |= 4;
-                    if (((L.cls >> 5) & 7) < 7) L.cls += 32;  // contributions, saturating at 7
-                }
-            }
-            if (r_now > H.R_esc) {  // escape: sky along normalize(dx/dl) (kerr_trace.cuh:229-237)
-                bool grid = false;
-                double sk = sky(P, ns, L.pt, L.Tr, grid);
-                cnt.rhs++;
-                L.col[0] = L.col[0] + sk;
-                L.col[1] = L.col[1] + sk;
-                L.col[2] = L.col[2] + sk;
-                L.cls = (L.cls & ~3) | 2 | (grid ? 8 : 0);
-                done = true;
-            } else if (!(__builtin_isfinite(ns.x) && __builtin_isfinite(ns.y) && __builtin_isfinite(ns.z) &&
-                         __builtin_isfinite(ns.px) && __builtin_isfinite(ns.py) && __builtin_isfinite(ns.pz))) {
-                done = true;  // a non-finite state can add no further colour
-            }
-        }
         if (accepted) L.k1 = k7;  // FSAL
         L.have_k1 = accepted;
-        if (!done) {
-            L.step += 1;
-            if (L.step < H.max_steps && L.Tr > H.cutoff) {
-                L.tol = step_tolerance(L.s, r_now, g, H.tol, H.tol_floor);
-            } else {
-                done = true;
-            }
-        }
-        if (done) {
-            cnt.max_att = max(cnt.max_att, L.ray_att);
-            end_sample(L, P, H);
-        }
+        commit(L, s, ns, hs, H);
     }
 
     // ---- wave-level reduction of the work counters, one atomic per wave

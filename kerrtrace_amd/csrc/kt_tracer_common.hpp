@@ -93,17 +93,26 @@ __device__ __forceinline__ double disk_factor(const RenderParams& P, double u) {
     return c0 + w * (c1 - c0);
 }
 
-// Emission of one equatorial crossing (cuda/kerr_trace.cuh:174-226).
-// Returns true when colour was composited.
-static __device__ __noinline__ bool shade_crossing(const RenderParams& P, double x_hit, double z_hit, double px_hit,
-                                            double py_hit, double pz_hit, double pt, double kt, double col[3],
-                                            double& Tr, bool& overflow, double& overflow_T) {
+// Emission of one equatorial crossing (cuda/kerr_trace.cuh:174-226).  The
+// lane's colour and transmittance stay in the caller's registers: the
+// function returns the colour to add, d_col * alpha * Tr, and the factor
+// 1 - alpha the caller multiplies into Tr (passing lane state by pointer
+// into a non-inlined call would move the whole lane into scratch memory).
+struct Shade {
+    double r, g, b;       // colour to add
+    double tr_mul;        // transmittance factor
+    double overflow_T;    // temperature above lut_max_temp, or 0
+    int composited;
+};
+static __device__ __noinline__ Shade shade_crossing(const RenderParams& P, double x_hit, double z_hit, double px_hit,
+                                                    double py_hit, double pz_hit, double pt, double kt, double Tr) {
+    Shade o{0.0, 0.0, 0.0, 1.0, 0.0, 0};
     const double a = P.a, M = P.M, aa = P.aa;
     double r_hit = ks_r(x_hit, 0.0, z_hit, aa);
-    if (!(r_hit >= P.disk_inner && r_hit <= P.disk_outer)) return false;
+    if (!(r_hit >= P.disk_inner && r_hit <= P.disk_outer)) return o;
     double sqrt_M = sqrt(M), r_sqrt = sqrt(r_hit);
     double disc_denom = 1.0 - 3.0 * M / r_hit + 2.0 * a * sqrt_M / (r_hit * r_sqrt);
-    if (!(disc_denom > 0.0)) return false;
+    if (!(disc_denom > 0.0)) return o;
     double omega = -sqrt_M / (r_hit * r_sqrt + a * sqrt_M);
     double r2_hit = r_hit * r_hit;
     double denom_hit = r2_hit + aa;
@@ -154,27 +163,50 @@ static __device__ __noinline__ bool shade_crossing(const RenderParams& P, double
         double modulator = 1.0 + P.noise_strength * (nval - 0.5);
         T = T * modulator;
     }
-    if (!(__builtin_isfinite(T) && T > 0.0)) return false;
+    if (!(__builtin_isfinite(T) && T > 0.0)) return o;
     double d_col[3];
     bool ovf = false;
     blackbody(P, T, d_col, ovf);
-    if (ovf && !overflow) {
-        overflow = true;
-        overflow_T = T;
-    }
+    if (ovf) o.overflow_T = T;
     double luma = 0.2126 * d_col[0] + 0.7152 * d_col[1] + 0.0722 * d_col[2];
     double luma_scaled = luma / P.wl_step;
     double alpha = 1.0 - exp(-luma_scaled * 0.5);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) col[c] = col[c] + d_col[c] * alpha * Tr;
-    Tr = Tr * (1.0 - alpha);
-    return alpha > 0.0;
+    o.r = d_col[0] * alpha * Tr;
+    o.g = d_col[1] * alpha * Tr;
+    o.b = d_col[2] * alpha * Tr;
+    o.tr_mul = 1.0 - alpha;
+    o.composited = alpha > 0.0 ? 1 : 0;
+    return o;
 }
 
-// Sky grid intensity along the escape direction (cuda/kerr_math.cuh:157-171).
-static __device__ __noinline__ double sky(const RenderParams& P, const V6& s, double pt, double Tr, bool& grid) {
+// Applies a crossing's emission to the lane's colour, transmittance and
+// class (same operations as cuda/kerr_trace.cuh:221-224).
+__device__ __forceinline__ void apply_shade(const Shade& sh, double col[3], double& Tr, int& cls, bool& overflow,
+                                            double& overflow_T) {
+    col[0] = col[0] + sh.r;
+    col[1] = col[1] + sh.g;
+    col[2] = col[2] + sh.b;
+    Tr = Tr * sh.tr_mul;
+    if (sh.overflow_T > 0.0 && !overflow) {
+        overflow = true;
+        overflow_T = sh.overflow_T;
+    }
+    if (sh.composited) {
+        cls |= 4;
+        if (((cls >> 5) & 7) < 7) cls += 32;  // contributions, saturating at 7
+    }
+}
+
+// Sky grid intensity along the escape direction (cuda/kerr_math.cuh:157-171);
+// the state arrives by value for the same reason as in shade_crossing.
+struct Sky {
+    double intensity;
+    int grid;
+};
+static __device__ __noinline__ Sky sky(const RenderParams& P, double x, double y, double z, double px, double py,
+                                       double pz, double pt, double Tr) {
     Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
-    V6 ke = rhs(s, pt, g);
+    V6 ke = rhs(V6{x, y, z, px, py, pz}, pt, g);
     double q = ke.x * ke.x + ke.y * ke.y + ke.z * ke.z;
     double inv = 1.0 / sqrt(q);
     double dx = ke.x * inv, dy = ke.y * inv, dz = ke.z * inv;
@@ -184,9 +216,60 @@ static __device__ __noinline__ double sky(const RenderParams& P, const V6& s, do
     double lt = P.sky_lt;
     double d_theta = fmod(fabs(theta), spacing);
     double d_phi = fmod(fabs(phi_lat), spacing);
-    grid = (d_theta < lt || d_theta > (spacing - lt)) || (d_phi < lt || d_phi > (spacing - lt)) ||
-           (fabs(phi_lat) < lt * 3.0);
-    return grid ? P.sky_int * Tr : 0.0;
+    bool grid = (d_theta < lt || d_theta > (spacing - lt)) || (d_phi < lt || d_phi > (spacing - lt)) ||
+                (fabs(phi_lat) < lt * 3.0);
+    return Sky{grid ? P.sky_int * Tr : 0.0, grid ? 1 : 0};
+}
+
+// One committed step that crossed the equator (cuda/kerr_trace.cuh:138-226),
+// recomputed from its start state s and step size hs: k1 = rhs(s) (equal, bit
+// for bit, to the FSAL derivative the step used), k2..k7 and the 5th-order
+// end point with the same operations as the tracer's attempt, then the
+// crossing parameter on the dense output (secant as written, or the FP64
+// root), the dense-output hit state and its emission.  Recomputing here
+// keeps the six stage derivatives out of the attempt's live registers.
+__device__ __forceinline__ Shade cross_step(const RenderParams& P, double sx, double sy, double sz, double spx,
+                                                double spy, double spz, double hs, double pt, double kt, double Tr) {
+    const Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
+    const V6 s{sx, sy, sz, spx, spy, spz};
+    const V6 k1 = rhs(s, pt, g);
+#define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
+#define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
+#define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
+#define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
+#define KT_ST6(c) \
+    fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
+#define KT_ST7(c) \
+    fma(hs, fma(KT_B6, k6.c, fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))), s.c)
+#define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
+    const V6 k2 = rhs(KT_STAGE(KT_ST2), pt, g);
+    const V6 k3 = rhs(KT_STAGE(KT_ST3), pt, g);
+    const V6 k4 = rhs(KT_STAGE(KT_ST4), pt, g);
+    const V6 k5 = rhs(KT_STAGE(KT_ST5), pt, g);
+    const V6 k6 = rhs(KT_STAGE(KT_ST6), pt, g);
+    const V6 ns = KT_STAGE(KT_ST7);
+    const V6 k7 = rhs(ns, pt, g);
+#undef KT_ST2
+#undef KT_ST3
+#undef KT_ST4
+#undef KT_ST5
+#undef KT_ST6
+#undef KT_ST7
+#undef KT_STAGE
+    double t;
+    if (P.disk_root_mode == 1) {
+        t = disk_root(s.y, ns.y, k1.y, k3.y, k4.y, k5.y, k6.y, k7.y, hs);
+    } else {
+        double den = s.y - ns.y;
+        t = den != 0.0 ? s.y / den : 0.5;
+        t = fmin(fmax(t, 0.0), 1.0);
+    }
+    const double x_hit = dense(s.x, ns.x, k1.x, k3.x, k4.x, k5.x, k6.x, k7.x, hs, t);
+    const double z_hit = dense(s.z, ns.z, k1.z, k3.z, k4.z, k5.z, k6.z, k7.z, hs, t);
+    const double px_hit = dense(s.px, ns.px, k1.px, k3.px, k4.px, k5.px, k6.px, k7.px, hs, t);
+    const double py_hit = dense(s.py, ns.py, k1.py, k3.py, k4.py, k5.py, k6.py, k7.py, hs, t);
+    const double pz_hit = dense(s.pz, ns.pz, k1.pz, k3.pz, k4.pz, k5.pz, k6.pz, k7.pz, hs, t);
+    return shade_crossing(P, x_hit, z_hit, px_hit, py_hit, pz_hit, pt, kt, Tr);
 }
 
 struct LaneCounters {
@@ -236,8 +319,13 @@ struct Lane {
     double col[3], acc[3];
     double overflow_T;
     int rid, pix_x, pix_y, sample, step, att_in_step, cls;
+    V6 s0;            // start state of the last committed step
+    double r_now;     // Kerr-Schild radius of the committed state
+    double h_used;    // step size of the last committed step
     uint32_t ray_att;
-    bool active, have_k1, overflow;
+    bool active, finished, have_k1, overflow;
+    bool post;        // the committed step still needs its crossing/escape/budget pass
+    bool crossing;    // ... and it crossed the equator
 };
 
 // Hot per-render values a lane keeps in registers.
@@ -311,18 +399,19 @@ __device__ __forceinline__ void start_sample(Lane& L, const RenderParams& P, con
     L.tol = step_tolerance(L.s, ks_r(L.s.x, L.s.y, L.s.z, g.aa), g, H.tol, H.tol_floor);
 }
 
-// A sample ended: accumulate it, then start the next sample of the pixel
-// or write the pixel (cuda/kernel.cu:132-141) and free the lane.  Samples
-// whose step budget is empty (max_steps = 0 or cutoff >= 1) add nothing.
+// A sample ended: accumulate it, then start the pixel's next sample (a
+// sample whose step budget is empty is marked finished again and ends on
+// the next pass) or write the pixel (cuda/kernel.cu:132-141) and free the
+// lane.
 __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const Hot& H) {
-    for (;;) {
-        L.acc[0] = L.acc[0] + L.col[0];
-        L.acc[1] = L.acc[1] + L.col[1];
-        L.acc[2] = L.acc[2] + L.col[2];
-        L.sample += 1;
-        if (L.sample >= H.spp) break;
+    L.acc[0] = L.acc[0] + L.col[0];
+    L.acc[1] = L.acc[1] + L.col[1];
+    L.acc[2] = L.acc[2] + L.col[2];
+    L.sample += 1;
+    if (L.sample < H.spp) {
         start_sample(L, P, H);
-        if (0 < H.max_steps && L.Tr > H.cutoff) return;
+        L.finished = !(0 < H.max_steps && L.Tr > H.cutoff);
+        return;
     }
     double fs = P.exposure_over_spp;
     size_t idx = (size_t)L.pix_y * (size_t)P.W + (size_t)L.pix_x;
@@ -334,6 +423,98 @@ __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const
         if (atomicCAS(P.error_flag, 0u, 1u) == 0u) P.error_flag[1] = __float_as_uint((float)L.overflow_T);
     }
     L.active = false;
+}
+
+// Second half of a commit (cuda/kerr_trace.cuh:138-237 after the horizon
+// test): the equatorial crossing's emission, the escape test with the sky
+// colour, the non-finite stop, then the step counter and the loop condition
+// (kerr_trace.cuh:62) with the next step's tolerance.  It runs at the top of
+// the tracer's next iteration, where no stage derivative is live.
+__device__ __forceinline__ void post_commit(Lane& L, const RenderParams& P, const Hot& H, LaneCounters& cnt) {
+    if (L.crossing) {
+        cnt.crossings++;
+        const Shade sh = cross_step(P, L.s0.x, L.s0.y, L.s0.z, L.s0.px, L.s0.py, L.s0.pz, L.h_used, L.pt, L.kt, L.Tr);
+        apply_shade(sh, L.col, L.Tr, L.cls, L.overflow, L.overflow_T);
+    }
+    const V6& ns = L.s;
+    bool done = false;
+    if (L.r_now > H.R_esc) {  // escape: sky along normalize(dx/dl) (kerr_trace.cuh:229-237)
+        const Sky sk = sky(P, ns.x, ns.y, ns.z, ns.px, ns.py, ns.pz, L.pt, L.Tr);
+        cnt.rhs++;
+        L.col[0] = L.col[0] + sk.intensity;
+        L.col[1] = L.col[1] + sk.intensity;
+        L.col[2] = L.col[2] + sk.intensity;
+        L.cls = (L.cls & ~3) | 2 | (sk.grid ? 8 : 0);
+        done = true;
+    } else if (!(__builtin_isfinite(ns.x) && __builtin_isfinite(ns.y) && __builtin_isfinite(ns.z) &&
+                 __builtin_isfinite(ns.px) && __builtin_isfinite(ns.py) && __builtin_isfinite(ns.pz))) {
+        done = true;  // a non-finite state can add no further colour
+    }
+    if (!done) {
+        L.step += 1;
+        if (L.step < H.max_steps && L.Tr > H.cutoff) {
+            L.tol = step_tolerance(L.s, L.r_now, H.g, H.tol, H.tol_floor);
+        } else {
+            done = true;
+        }
+    }
+    L.finished = done;
+}
+
+// First half of a commit (kerr_trace.cuh:123-137): the attempt's end point
+// becomes the state (also after max_attempts rejections), then the horizon
+// test; everything else is left to post_commit.
+__device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, double hs, const Hot& H) {
+    L.s0 = s;
+    L.s = ns;
+    L.att_in_step = 0;
+    L.h_used = hs;
+    L.r_now = ks_r(ns.x, ns.y, ns.z, H.g.aa);
+    if (L.r_now < H.rh_eps) {  // horizon, tested before the disk (kerr_trace.cuh:135-137)
+        L.cls = (L.cls & ~3) | 1;
+        L.finished = true;
+        return;
+    }
+    L.crossing = (s.y < 0.0 && ns.y >= 0.0) || (s.y > 0.0 && ns.y <= 0.0);
+    L.post = true;
+}
+
+// Refill the wave's idle lanes from the work queue: one atomic per wave
+// claims as many items as there are idle lanes once `refill_threshold` of
+// them are idle (or all are), and ballot + popcount hand them out.  Items
+// past the end or outside their image leave the lane idle.  A new ray whose
+// step budget is empty is marked finished right away.
+template <bool BATCH>
+__device__ __forceinline__ void refill(Lane& L, Hot& H, const RenderParams* __restrict__ Ps, int n_renders,
+                                       uint64_t total_items, int refill_threshold, DevCounters* __restrict__ counters,
+                                       int lane, bool& drained) {
+    uint64_t idle = __ballot(!L.active);
+    if (idle == 0 || drained) return;
+    int n_idle = __popcll(idle);
+    if (n_idle < refill_threshold && idle != __ballot(1)) return;
+    int leader = __ffsll((unsigned long long)idle) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&counters->queue_head, (unsigned long long)n_idle);
+    base = __shfl(base, leader);
+    if (base + (unsigned long long)n_idle >= total_items) drained = true;
+    if (L.active) return;
+    uint64_t item = base + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
+    int rid = 0, px = 0, py = 0;
+    if (item < total_items && decode_item<BATCH>(Ps, n_renders, item, rid, px, py)) {
+        L.rid = rid;
+        L.pix_x = px;
+        L.pix_y = py;
+        const RenderParams& P = prm<BATCH>(Ps, rid);
+        if (BATCH) H = load_hot(P);
+        L.sample = 0;
+        L.acc[0] = L.acc[1] = L.acc[2] = 0.0;
+        L.overflow = false;
+        L.ray_att = 0;
+        L.active = true;
+        L.post = false;
+        start_sample(L, P, H);
+        L.finished = !(0 < H.max_steps && L.Tr > H.cutoff);
+    }
 }
 
 }  // namespace kt
