@@ -109,6 +109,9 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
             "trace_mrays_s": round(rays / t / 1e6, 4),
             "speedup_vs_fp64": round(t64 / t, 4),
             "attempts_per_ray": round(st["attempts"] / rays, 3),
+            "max_attempts_ray": st["max_attempts_ray"],
+            "kernel_ms": round(st["kernel_us"] / 1e3, 4),
+            "tail_ms": round(st["tail_us"] / 1e3, 4),
             "algorithmic_tflops": round(trace_flops(st, fc) / t / 1e12, 4),
             "frac_of_fp32_vector_peak": round(trace_flops(st, fc) / t / PEAK_FP32_VECTOR, 4),
             "vs_fp64_frame": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac", "class_mismatch_non_boundary",
@@ -198,7 +201,8 @@ def main():
         "wall_s": round(wall, 3),
         "work": {"rhs_evals": st["rhs_evals"], "attempts": st["attempts"], "accepted": st["accepted"],
                  "crossings": st["crossings"], "max_attempts_ray": st["max_attempts_ray"],
-                 "attempts_per_ray": round(st["attempts"] / rays, 3)},
+                 "attempts_per_ray": round(st["attempts"] / rays, 3),
+                 "kernel_ms": round(st["kernel_us"] / 1e3, 4), "tail_ms": round(st["tail_us"] / 1e3, 4)},
         "roofline": {"bound": "valu-fp64", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4), "traffic": None,
                      "kernel": "trace_fp64_persistent",

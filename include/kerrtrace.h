@@ -180,6 +180,8 @@ typedef struct kt_trace_stats {
     uint64_t max_attempts_ray; /* largest attempt count of any single ray */
     uint32_t lut_overflow;  /* 1 if a disk temperature exceeded lut_max_temp */
     float overflow_temperature; /* the first such temperature (kernel.cu:5-8) */
+    double kernel_us;       /* first wave start to last wave exit (device clock) */
+    double tail_us;         /* work queue empty to last wave exit */
 } kt_trace_stats;
 
 /* Tuning knobs of the persistent tracer (new).  Zero means "default". */
@@ -232,6 +234,11 @@ kt_status kt_finish_frame(kt_renderer* r, int32_t index);
 /* hdr: float4[W*H] (rgb, 1); hit_class: uint8[W*H] or NULL. */
 kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr,
                    uint8_t* hit_class, void* stream);
+/* As kt_trace, also writing each pixel's DP5(4) attempt count to
+ * attempts (uint32[W*H], may be NULL).  New: the reference has no
+ * per-ray work output. */
+kt_status kt_trace_ex(kt_renderer* r, const kt_camera* cam, float* hdr,
+                      uint8_t* hit_class, uint32_t* attempts, void* stream);
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* opts);
 /* Synchronises the renderer stream and returns the last trace's counters. */
 kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out);

@@ -124,6 +124,8 @@ class KtTraceStats(C.Structure):
         ("max_attempts_ray", C.c_uint64),
         ("lut_overflow", C.c_uint32),
         ("overflow_temperature", C.c_float),
+        ("kernel_us", C.c_double),
+        ("tail_us", C.c_double),
     ]
 
 

@@ -30,6 +30,7 @@ SIGNATURES = {
     "kt_ready_frame": ([_vp, C.POINTER(KtFrameView), C.POINTER(_i32)], _i32),
     "kt_finish_frame": ([_vp, _i32], _i32),
     "kt_trace": ([_vp, C.POINTER(KtCamera), _vp, _vp, _vp], _i32),
+    "kt_trace_ex": ([_vp, C.POINTER(KtCamera), _vp, _vp, _vp, _vp], _i32),
     "kt_set_launch_opts": ([_vp, C.POINTER(KtLaunchOpts)], _i32),
     "kt_get_trace_stats": ([_vp, C.POINTER(KtTraceStats)], _i32),
     "kt_post_process": ([_vp, _vp, _vp, _vp, _vp], _i32),

@@ -308,7 +308,7 @@ void release(kt_renderer* r) {
 
 // The parameter record of one render with camera `cam` writing to hdr/cls.
 void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, const kt_camera& cam, float* hdr,
-                 uint8_t* cls, uint32_t* flag, int tile, kt::RenderParams& P) {
+                 uint8_t* cls, uint32_t* flag, int tile, kt::RenderParams& P, uint32_t* attempts = nullptr) {
     std::memset(&P, 0, sizeof P);
     P.a = c.spin;
     P.M = c.mass;
@@ -349,6 +349,7 @@ void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, con
     P.bb_lut_f = r ? r->d_bb_lut_f : nullptr;
     P.hdr = hdr;
     P.cls = cls;
+    P.attempts_out = attempts;
     P.error_flag = flag;
     P.W = (int)c.width;
     P.H = (int)c.height;
@@ -381,10 +382,11 @@ int upload_params(kt_renderer* r, const kt::RenderParams& P, hipStream_t s, kt::
     return KT_OK;
 }
 
-int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* flag, hipStream_t s) {
+int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* flag, hipStream_t s,
+               uint32_t* attempts = nullptr) {
     kt::RenderParams P;
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
-    make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P);
+    make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
     kt::RenderParams* dp = nullptr;
     int rc = upload_params(r, P, s, &dp);
     if (rc != KT_OK) return rc;
@@ -657,6 +659,12 @@ kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cl
     return trace_into(r, cam, hdr, cls, r->d_trace_flag, (hipStream_t)stream);
 }
 
+kt_status kt_trace_ex(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* attempts,
+                      void* stream) {
+    if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    return trace_into(r, cam, hdr, cls, r->d_trace_flag, (hipStream_t)stream, attempts);
+}
+
 kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
     if (!r || !out) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     KT_HIP(r, hipStreamSynchronize(r->last_stream), "stream sync");
@@ -674,6 +682,15 @@ kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
     float T;
     std::memcpy(&T, &flag[1], sizeof T);
     out->overflow_temperature = flag[0] ? T : 0.0f;
+    // device wall clock ticks -> microseconds
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, r->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    const double us_per_tick = 1000.0 / (double)rate_khz;
+    out->kernel_us = c.clock_end > c.clock_start ? (double)(c.clock_end - c.clock_start) * us_per_tick : 0.0;
+    out->tail_us = (c.clock_drain != ~0ull && c.clock_end > c.clock_drain)
+                       ? (double)(c.clock_end - c.clock_drain) * us_per_tick
+                       : 0.0;
     return KT_OK;
 }
 

@@ -33,6 +33,7 @@ struct RenderParams {
     // outputs
     float* hdr;               // float4 per pixel
     uint8_t* cls;             // may be null
+    uint32_t* attempts_out;   // per-pixel DP5(4) attempts, may be null
     uint32_t* error_flag;     // [0] set once, [1] float bits of the first overflowing T
     int32_t W, H, spp, max_steps, max_attempts, lut_n, noise_detail, disk_root_mode;
     int32_t rhs_exact;        // 1: exact Hamiltonian dr^2/dy
@@ -51,7 +52,9 @@ struct DevCounters {
     unsigned long long crossings;
     unsigned long long max_attempts_ray;
     unsigned long long queue_head;   // persistent work-queue head
-    unsigned long long reserved[2];
+    unsigned long long clock_start;  // earliest wave start (wall clock ticks)
+    unsigned long long clock_drain;  // earliest time a wave found the queue empty
+    unsigned long long clock_end;    // latest wave exit
 };
 
 }  // namespace kt

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     LaneCounters cnt;
     bool drained = false;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
+    if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
     for (;;) {
         if (L.post) {
@@ -214,23 +215,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         commit(L, s0, ns, h, H);
     }
 
-    unsigned long long r = cnt.rhs, a = cnt.attempts, c = cnt.accepted, x = cnt.crossings, m = cnt.max_att;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        r += __shfl_xor(r, off);
-        a += __shfl_xor(a, off);
-        c += __shfl_xor(c, off);
-        x += __shfl_xor(x, off);
-        unsigned long long mo = __shfl_xor(m, off);
-        m = mo > m ? mo : m;
-    }
-    if (lane == 0) {
-        atomicAdd(&counters->rhs_evals, r);
-        atomicAdd(&counters->attempts, a);
-        atomicAdd(&counters->accepted, c);
-        atomicAdd(&counters->crossings, x);
-        atomicMax(&counters->max_attempts_ray, m);
-    }
+    flush_counters(cnt, counters, lane);
 }
 
 hipError_t launch_trace_mixed(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,

@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     LaneCounters cnt;
     bool drained = false;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
+    if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
     for (;;) {
         // ---- second half of last iteration's commit, then rays that ended:
@@ -137,24 +138,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         commit(L, s, ns, hs, H);
     }
 
-    // ---- wave-level reduction of the work counters, one atomic per wave
-    unsigned long long r = cnt.rhs, a = cnt.attempts, c = cnt.accepted, x = cnt.crossings, m = cnt.max_att;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        r += __shfl_xor(r, off);
-        a += __shfl_xor(a, off);
-        c += __shfl_xor(c, off);
-        x += __shfl_xor(x, off);
-        unsigned long long mo = __shfl_xor(m, off);
-        m = mo > m ? mo : m;
-    }
-    if (lane == 0) {
-        atomicAdd(&counters->rhs_evals, r);
-        atomicAdd(&counters->attempts, a);
-        atomicAdd(&counters->accepted, c);
-        atomicAdd(&counters->crossings, x);
-        atomicMax(&counters->max_attempts_ray, m);
-    }
+    flush_counters(cnt, counters, lane);
 }
 
 __global__ void reset_counters(DevCounters* c, uint32_t* error_flag) {
@@ -165,6 +149,9 @@ __global__ void reset_counters(DevCounters* c, uint32_t* error_flag) {
         c->crossings = 0;
         c->max_attempts_ray = 0;
         c->queue_head = 0;
+        c->clock_start = ~0ull;
+        c->clock_drain = ~0ull;
+        c->clock_end = 0;
         if (error_flag) {
             error_flag[0] = 0;
             error_flag[1] = 0;

@@ -419,6 +419,7 @@ __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const
     *reinterpret_cast<float4*>(P.hdr + idx * 4) = out;
     int c = L.cls | (L.overflow ? 16 : 0);
     if (P.cls) P.cls[idx] = (uint8_t)c;
+    if (P.attempts_out) P.attempts_out[idx] = L.ray_att;
     if (L.overflow && P.error_flag) {
         if (atomicCAS(P.error_flag, 0u, 1u) == 0u) P.error_flag[1] = __float_as_uint((float)L.overflow_T);
     }
@@ -479,6 +480,30 @@ __device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, doubl
     L.post = true;
 }
 
+// Wave-level reduction of the work counters at kernel exit: one atomic per
+// counter per wave, plus the wave's exit time.
+__device__ __forceinline__ void flush_counters(const LaneCounters& cnt, DevCounters* __restrict__ counters,
+                                               int lane) {
+    unsigned long long r = cnt.rhs, a = cnt.attempts, c = cnt.accepted, x = cnt.crossings, m = cnt.max_att;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off);
+        a += __shfl_xor(a, off);
+        c += __shfl_xor(c, off);
+        x += __shfl_xor(x, off);
+        unsigned long long mo = __shfl_xor(m, off);
+        m = mo > m ? mo : m;
+    }
+    if (lane == 0) {
+        atomicAdd(&counters->rhs_evals, r);
+        atomicAdd(&counters->attempts, a);
+        atomicAdd(&counters->accepted, c);
+        atomicAdd(&counters->crossings, x);
+        atomicMax(&counters->max_attempts_ray, m);
+        atomicMax(&counters->clock_end, (unsigned long long)wall_clock64());
+    }
+}
+
 // Refill the wave's idle lanes from the work queue: one atomic per wave
 // claims as many items as there are idle lanes once `refill_threshold` of
 // them are idle (or all are), and ballot + popcount hand them out.  Items
@@ -496,7 +521,10 @@ __device__ __forceinline__ void refill(Lane& L, Hot& H, const RenderParams* __re
     unsigned long long base = 0;
     if (lane == leader) base = atomicAdd(&counters->queue_head, (unsigned long long)n_idle);
     base = __shfl(base, leader);
-    if (base + (unsigned long long)n_idle >= total_items) drained = true;
+    if (base + (unsigned long long)n_idle >= total_items) {
+        drained = true;
+        if (lane == leader) atomicMin(&counters->clock_drain, (unsigned long long)wall_clock64());
+    }
     if (L.active) return;
     uint64_t item = base + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
     int rid = 0, px = 0, py = 0;

@@ -75,9 +75,10 @@ class HipRenderer:
         torch = _torch()
         return torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
 
-    def trace(self, cam: KtCamera, hdr=None, cls=None, with_class=True, stream=None):
+    def trace(self, cam: KtCamera, hdr=None, cls=None, with_class=True, stream=None, attempts=None):
         """Trace one frame on the current torch stream.  Returns (hdr, cls):
-        float32 [H, W, 4] RGBA (A = 1) and uint8 [H, W] hit classes."""
+        float32 [H, W, 4] RGBA (A = 1) and uint8 [H, W] hit classes.
+        attempts: optional int32 [H, W] tensor receiving per-pixel attempts."""
         torch = _torch()
         if hdr is None:
             hdr = self.alloc_hdr()
@@ -87,6 +88,12 @@ class HipRenderer:
         if cls is not None:
             _check_tensor(cls, (self.height, self.width), torch.uint8)
         s = stream if stream is not None else torch.cuda.current_stream()
+        if attempts is not None:
+            _check_tensor(attempts, (self.height, self.width), torch.int32)
+            check(lib().kt_trace_ex(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()),
+                                    C.c_void_p(cls.data_ptr() if cls is not None else 0),
+                                    C.c_void_p(attempts.data_ptr()), C.c_void_p(s.cuda_stream)), self._h)
+            return hdr, cls
         check(lib().kt_trace(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()),
                              C.c_void_p(cls.data_ptr() if cls is not None else 0),
                              C.c_void_p(s.cuda_stream)), self._h)
