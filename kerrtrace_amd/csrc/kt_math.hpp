@@ -278,6 +278,57 @@ __device__ __forceinline__ double dense_dt(double y0, double y1, double k1, doub
     return (r2 + r3) - 2.0 * t * (r3 - r4 - r5) - 3.0 * t2 * (r4 + 2.0 * r5) + 4.0 * t3 * r5;
 }
 
+// The same interpolant from its three step-dependent ingredients: k1, k7
+// and dsum = d1 k1 + d3 k3 + d4 k4 + d5 k5 + d6 k6 + d7 k7 (summed left to
+// right, as dense() does), so a tracer can fold the sum while it runs the
+// stages instead of keeping every stage derivative.
+__device__ __forceinline__ double dense_s(double y0, double y1, double k1, double k7, double dsum, double h,
+                                          double t) {
+    double r1 = y0;
+    double r2 = y1 - y0;
+    double r3 = y0 + h * k1 - y1;
+    double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
+    double r5 = h * dsum;
+    double t2 = t * t;
+    double t3 = t2 * t;
+    double t4 = t2 * t2;
+    return r1 + t * (r2 + r3) - t2 * (r3 - r4 - r5) - t3 * (r4 + 2.0 * r5) + t4 * r5;
+}
+__device__ __forceinline__ double dense_s_dt(double y0, double y1, double k1, double k7, double dsum, double h,
+                                             double t) {
+    double r2 = y1 - y0;
+    double r3 = y0 + h * k1 - y1;
+    double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
+    double r5 = h * dsum;
+    double t2 = t * t;
+    double t3 = t2 * t;
+    return (r2 + r3) - 2.0 * t * (r3 - r4 - r5) - 3.0 * t2 * (r4 + 2.0 * r5) + 4.0 * t3 * r5;
+}
+// disk_root on dense_s (identical iteration).
+__device__ __forceinline__ double disk_root_s(double y0, double y1, double k1, double k7, double dsum, double h) {
+    double lo = 0.0, hi = 1.0, ylo = y0;
+    double den = y0 - y1;
+    double t = den != 0.0 ? y0 / den : 0.5;
+    t = fmin(fmax(t, 0.0), 1.0);
+    for (int it = 0; it < 64; ++it) {
+        double yt = dense_s(y0, y1, k1, k7, dsum, h, t);
+        if (yt == 0.0) return t;
+        if ((yt >= 0.0) == (ylo >= 0.0)) {
+            lo = t;
+            ylo = yt;
+        } else {
+            hi = t;
+        }
+        double dyt = dense_s_dt(y0, y1, k1, k7, dsum, h, t);
+        double tn = dyt != 0.0 ? t - yt / dyt : 0.5 * (lo + hi);
+        if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        double step = tn - t;
+        t = tn;
+        if (fabs(step) <= 1e-15 || hi - lo <= 1e-15) break;
+    }
+    return t;
+}
+
 // Root of the interpolant y(t) on [0,1]: secant start, Newton steps kept
 // inside the shrinking bracket, bisection when Newton leaves it.
 __device__ __forceinline__ double disk_root(double y0, double y1, double k1, double k3, double k4, double k5,

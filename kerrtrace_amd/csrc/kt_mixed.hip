@@ -19,8 +19,8 @@
 // so FP32 rounding of the right-hand side does not set a noise floor on
 // the error norm even at tolerance 1e-9.  FP64 is kept for: the state and
 // its accumulation, the Kerr-Schild radius solve, the error norm and step
-// controller, the dense-output disk root (the crossing step is redone in
-// FP64 from its start state) and shading.
+// controller, the dense-output disk root (FP64 interpolant rebuilt from the
+// step's FP32 pieces) and shading.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -58,6 +58,19 @@ namespace kt {
 #define EA4 F(KT_E5 * KT_A54 + KT_E6 * KT_A64 + KT_E7 * KT_B4)
 #define EA5 F(KT_E6 * KT_A65 + KT_E7 * KT_B5)
 #define EA6 F(KT_E7 * KT_B6)
+// dense-output weights: (d A)_k = sum_j d_j a_jk (a_7k = b_k); sum_j d_j = 0
+#define DD1 Dense::d1
+#define DD3 Dense::d3
+#define DD4 Dense::d4
+#define DD5 Dense::d5
+#define DD6 Dense::d6
+#define DD7 Dense::d7
+#define DA1 F(DD3 * KT_A31 + DD4 * KT_A41 + DD5 * KT_A51 + DD6 * KT_A61 + DD7 * KT_B1)
+#define DA2 F(DD3 * KT_A32 + DD4 * KT_A42 + DD5 * KT_A52 + DD6 * KT_A62)
+#define DA3 F(DD4 * KT_A43 + DD5 * KT_A53 + DD6 * KT_A63 + DD7 * KT_B3)
+#define DA4 F(DD5 * KT_A54 + DD6 * KT_A64 + DD7 * KT_B4)
+#define DA5 F(DD6 * KT_A65 + DD7 * KT_B5)
+#define DA6 F(DD7 * KT_B6)
 
 struct X64 {
     double x, y, z;
@@ -118,6 +131,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         V6 ns;
         K32 esum;     // sum_j dc_j k_j pieces (corr, kp)
         P32 eq;       // sum_k (dc A)_k kp_k
+        K32 dsum;     // sum_j d_j k_j pieces (dense output)
+        P32 dq;       // sum_k (d A)_k kp_k
+        const K32 k1s = k1m;
         {
             const K32 k1 = k1m;
             // stage input from FP64 x + h c p plus the FP32 pieces
@@ -153,6 +169,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #define SBA5(c) (F(KT_B1) * k1.c + F(KT_B3) * k3.c + F(KT_B4) * k4.c + F(KT_B5) * k5.c)
 #define SEQ5(c) (EA1 * k1.c + EA2 * k2.c + EA3 * k3.c + EA4 * k4.c + EA5 * k5.c)
 #define SEA5(c) (F(KT_E1) * k1.c + F(KT_E3) * k3.c + F(KT_E4) * k4.c + F(KT_E5) * k5.c)
+#define SDQ5(c) (DA1 * k1.c + DA2 * k2.c + DA3 * k3.c + DA4 * k4.c + DA5 * k5.c)
+#define SDA5(c) (F(DD1) * k1.c + F(DD3) * k3.c + F(DD4) * k4.c + F(DD5) * k5.c)
+            dq = P32{SDQ5(px), SDQ5(py), SDQ5(pz)};
+            dsum = K32{SDA5(cx), SDA5(cy), SDA5(cz), SDA5(px), SDA5(py), SDA5(pz)};
             const P32 bq{SBQ(px), SBQ(py), SBQ(pz)};
             K32 bsum{SBA5(cx), SBA5(cy), SBA5(cz), SBA5(px), SBA5(py), SBA5(pz)};
             eq = P32{SEQ5(px), SEQ5(py), SEQ5(pz)};
@@ -161,6 +181,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
             bsum = K32{fmaf(F(KT_B6), k6.cx, bsum.cx), fmaf(F(KT_B6), k6.cy, bsum.cy), fmaf(F(KT_B6), k6.cz, bsum.cz),
                        fmaf(F(KT_B6), k6.px, bsum.px), fmaf(F(KT_B6), k6.py, bsum.py), fmaf(F(KT_B6), k6.pz, bsum.pz)};
             eq = P32{fmaf(EA6, k6.px, eq.x), fmaf(EA6, k6.py, eq.y), fmaf(EA6, k6.pz, eq.z)};
+            dq = P32{fmaf(DA6, k6.px, dq.x), fmaf(DA6, k6.py, dq.y), fmaf(DA6, k6.pz, dq.z)};
+            dsum = K32{fmaf(F(DD6), k6.cx, dsum.cx), fmaf(F(DD6), k6.cy, dsum.cy), fmaf(F(DD6), k6.cz, dsum.cz),
+                       fmaf(F(DD6), k6.px, dsum.px), fmaf(F(DD6), k6.py, dsum.py), fmaf(F(DD6), k6.pz, dsum.pz)};
             esum = K32{fmaf(F(KT_E6), k6.cx, esum.cx), fmaf(F(KT_E6), k6.cy, esum.cy), fmaf(F(KT_E6), k6.cz, esum.cz),
                        fmaf(F(KT_E6), k6.px, esum.px), fmaf(F(KT_E6), k6.py, esum.py), fmaf(F(KT_E6), k6.pz, esum.pz)};
             // 5th-order solution (FP64 accumulation) = 7th stage input
@@ -185,11 +208,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #undef SBA5
 #undef SEQ5
 #undef SEA5
+#undef SDQ5
+#undef SDA5
         }
         const K32 k7 = rhs_mixed(ns.x, ns.y, ns.z, (float)ns.px, (float)ns.py, (float)ns.pz, ptf, aa, g32);
         cnt.rhs += 6;
         esum = K32{fmaf(F(KT_E7), k7.cx, esum.cx), fmaf(F(KT_E7), k7.cy, esum.cy), fmaf(F(KT_E7), k7.cz, esum.cz),
                    fmaf(F(KT_E7), k7.px, esum.px), fmaf(F(KT_E7), k7.py, esum.py), fmaf(F(KT_E7), k7.pz, esum.pz)};
+        dsum = K32{fmaf(F(DD7), k7.cx, dsum.cx), fmaf(F(DD7), k7.cy, dsum.cy), fmaf(F(DD7), k7.cz, dsum.cz),
+                   fmaf(F(DD7), k7.px, dsum.px), fmaf(F(DD7), k7.py, dsum.py), fmaf(F(DD7), k7.pz, dsum.pz)};
         // embedded error: FP32 sums of the gravitational pieces, FP64 norm
         const double tol = L.tol;
         double ex = fabs(h * (double)(hf * eq.x - esum.cx)) / (tol * fmax(fabs(X.x), 1.0));
@@ -206,13 +233,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         L.h = h * 0.9 * pow_m02(err);
         if (!accepted && L.att_in_step < H.max_attempts) continue;
 
-        // ---- commit (the crossing step is redone in FP64 in post_commit:
-        // dense output, disk root and shading all in FP64)
+        // ---- commit; the crossing's dense output, root and shading run in
+        // FP64 in post_commit from FP64 ingredients rebuilt here:
+        // dx/dl_j = p_j - corr_j, sum_j d_j p_j = h sum_k (d A)_k kp_k
         if (accepted) cnt.accepted++;
         if (accepted) k1m = k7;  // FSAL
         L.have_k1 = accepted;
         const V6 s0{X.x, X.y, X.z, Pd.x, Pd.y, Pd.z};
-        commit(L, s0, ns, h, H);
+        const V6 ck1{Pd.x - (double)k1s.cx, Pd.y - (double)k1s.cy, Pd.z - (double)k1s.cz,
+                     (double)k1s.px, (double)k1s.py, (double)k1s.pz};
+        const V6 ck7{ns.px - (double)k7.cx, ns.py - (double)k7.cy, ns.pz - (double)k7.cz,
+                     (double)k7.px, (double)k7.py, (double)k7.pz};
+        const V6 cd{h * (double)dq.x - (double)dsum.cx, h * (double)dq.y - (double)dsum.cy,
+                    h * (double)dq.z - (double)dsum.cz, (double)dsum.px, (double)dsum.py, (double)dsum.pz};
+        commit(L, s0, ns, h, ck1, ck7, cd, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
     }
 
     flush_counters(cnt, counters, lane);

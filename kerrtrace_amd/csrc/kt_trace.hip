@@ -75,7 +75,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         }
         const double pt = L.pt;
         const double hs = L.h;
-        V6 ns, eacc;
+        V6 ns, eacc, dacc;
+        const V6 k1s = L.k1;
         {
             const V6 k1 = L.k1;
             // stage inputs s + h * sum_j a_ij k_j as fma chains
@@ -98,11 +99,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #define KT_E5ACC(c) fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))
             V6 bacc = KT_STAGE(KT_B4ACC);
             eacc = KT_STAGE(KT_E5ACC);
+            // dense-output sum d1 k1 + d3 k3 + ... + d7 k7, left to right
+#define KT_D5ACC(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c)
+            dacc = KT_STAGE(KT_D5ACC);
             const V6 k6 = rhs(in6, pt, g);
 #define KT_NS(c) fma(hs, fma(KT_B6, k6.c, bacc.c), s.c)
 #define KT_EACC6(c) fma(KT_E6, k6.c, eacc.c)
             ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
             eacc = KT_STAGE(KT_EACC6);
+#define KT_D6ACC(c) (dacc.c + Dense::d6 * k6.c)
+            dacc = KT_STAGE(KT_D6ACC);
 #undef KT_ST2
 #undef KT_ST3
 #undef KT_ST4
@@ -110,11 +116,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #undef KT_ST6
 #undef KT_B4ACC
 #undef KT_E5ACC
+#undef KT_D5ACC
+#undef KT_D6ACC
 #undef KT_NS
 #undef KT_EACC6
         }
         const V6 k7 = rhs(ns, pt, g);
         cnt.rhs += 6;
+#define KT_D7ACC(c) (dacc.c + Dense::d7 * k7.c)
+        dacc = KT_STAGE(KT_D7ACC);
+#undef KT_D7ACC
         // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
         const double tol = L.tol;
 #define KT_ERR(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
@@ -135,7 +146,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         if (accepted) cnt.accepted++;
         if (accepted) L.k1 = k7;  // FSAL
         L.have_k1 = accepted;
-        commit(L, s, ns, hs, H);
+        commit(L, s, ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
     }
 
     flush_counters(cnt, counters, lane);

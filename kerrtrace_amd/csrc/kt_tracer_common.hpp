@@ -221,57 +221,6 @@ static __device__ __noinline__ Sky sky(const RenderParams& P, double x, double y
     return Sky{grid ? P.sky_int * Tr : 0.0, grid ? 1 : 0};
 }
 
-// One committed step that crossed the equator (cuda/kerr_trace.cuh:138-226),
-// recomputed from its start state s and step size hs: k1 = rhs(s) (equal, bit
-// for bit, to the FSAL derivative the step used), k2..k7 and the 5th-order
-// end point with the same operations as the tracer's attempt, then the
-// crossing parameter on the dense output (secant as written, or the FP64
-// root), the dense-output hit state and its emission.  Recomputing here
-// keeps the six stage derivatives out of the attempt's live registers.
-__device__ __forceinline__ Shade cross_step(const RenderParams& P, double sx, double sy, double sz, double spx,
-                                                double spy, double spz, double hs, double pt, double kt, double Tr) {
-    const Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
-    const V6 s{sx, sy, sz, spx, spy, spz};
-    const V6 k1 = rhs(s, pt, g);
-#define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
-#define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
-#define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
-#define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
-#define KT_ST6(c) \
-    fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
-#define KT_ST7(c) \
-    fma(hs, fma(KT_B6, k6.c, fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))), s.c)
-#define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
-    const V6 k2 = rhs(KT_STAGE(KT_ST2), pt, g);
-    const V6 k3 = rhs(KT_STAGE(KT_ST3), pt, g);
-    const V6 k4 = rhs(KT_STAGE(KT_ST4), pt, g);
-    const V6 k5 = rhs(KT_STAGE(KT_ST5), pt, g);
-    const V6 k6 = rhs(KT_STAGE(KT_ST6), pt, g);
-    const V6 ns = KT_STAGE(KT_ST7);
-    const V6 k7 = rhs(ns, pt, g);
-#undef KT_ST2
-#undef KT_ST3
-#undef KT_ST4
-#undef KT_ST5
-#undef KT_ST6
-#undef KT_ST7
-#undef KT_STAGE
-    double t;
-    if (P.disk_root_mode == 1) {
-        t = disk_root(s.y, ns.y, k1.y, k3.y, k4.y, k5.y, k6.y, k7.y, hs);
-    } else {
-        double den = s.y - ns.y;
-        t = den != 0.0 ? s.y / den : 0.5;
-        t = fmin(fmax(t, 0.0), 1.0);
-    }
-    const double x_hit = dense(s.x, ns.x, k1.x, k3.x, k4.x, k5.x, k6.x, k7.x, hs, t);
-    const double z_hit = dense(s.z, ns.z, k1.z, k3.z, k4.z, k5.z, k6.z, k7.z, hs, t);
-    const double px_hit = dense(s.px, ns.px, k1.px, k3.px, k4.px, k5.px, k6.px, k7.px, hs, t);
-    const double py_hit = dense(s.py, ns.py, k1.py, k3.py, k4.py, k5.py, k6.py, k7.py, hs, t);
-    const double pz_hit = dense(s.pz, ns.pz, k1.pz, k3.pz, k4.pz, k5.pz, k6.pz, k7.pz, hs, t);
-    return shade_crossing(P, x_hit, z_hit, px_hit, py_hit, pz_hit, pt, kt, Tr);
-}
-
 struct LaneCounters {
     uint32_t rhs = 0, attempts = 0, accepted = 0, crossings = 0, max_att = 0;
 };
@@ -319,9 +268,8 @@ struct Lane {
     double col[3], acc[3];
     double overflow_T;
     int rid, pix_x, pix_y, sample, step, att_in_step, cls;
-    V6 s0;            // start state of the last committed step
+    double hx, hz, hpx, hpy, hpz;  // dense-output state at the equatorial crossing
     double r_now;     // Kerr-Schild radius of the committed state
-    double h_used;    // step size of the last committed step
     uint32_t ray_att;
     bool active, finished, have_k1, overflow;
     bool post;        // the committed step still needs its crossing/escape/budget pass
@@ -434,7 +382,7 @@ __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const
 __device__ __forceinline__ void post_commit(Lane& L, const RenderParams& P, const Hot& H, LaneCounters& cnt) {
     if (L.crossing) {
         cnt.crossings++;
-        const Shade sh = cross_step(P, L.s0.x, L.s0.y, L.s0.z, L.s0.px, L.s0.py, L.s0.pz, L.h_used, L.pt, L.kt, L.Tr);
+        const Shade sh = shade_crossing(P, L.hx, L.hz, L.hpx, L.hpy, L.hpz, L.pt, L.kt, L.Tr);
         apply_shade(sh, L.col, L.Tr, L.cls, L.overflow, L.overflow_T);
     }
     const V6& ns = L.s;
@@ -462,14 +410,17 @@ __device__ __forceinline__ void post_commit(Lane& L, const RenderParams& P, cons
     L.finished = done;
 }
 
-// First half of a commit (kerr_trace.cuh:123-137): the attempt's end point
+// First half of a commit (kerr_trace.cuh:123-174): the attempt's end point
 // becomes the state (also after max_attempts rejections), then the horizon
-// test; everything else is left to post_commit.
-__device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, double hs, const Hot& H) {
-    L.s0 = s;
+// test, then -- on a change of sign of y -- the crossing parameter on the
+// step's DOPRI5 dense output (the clamped secant as written, or the FP64
+// Newton/bisection root) and the interpolated hit state.  k1, k7 and d
+// (sum_i d_i k_i) are the step's dense-output ingredients.  The emission
+// and the rest of the step are left to post_commit.
+__device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, double hs, const V6& k1, const V6& k7,
+                                       const V6& d, const Hot& H, int disk_root_mode) {
     L.s = ns;
     L.att_in_step = 0;
-    L.h_used = hs;
     L.r_now = ks_r(ns.x, ns.y, ns.z, H.g.aa);
     if (L.r_now < H.rh_eps) {  // horizon, tested before the disk (kerr_trace.cuh:135-137)
         L.cls = (L.cls & ~3) | 1;
@@ -477,6 +428,21 @@ __device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, doubl
         return;
     }
     L.crossing = (s.y < 0.0 && ns.y >= 0.0) || (s.y > 0.0 && ns.y <= 0.0);
+    if (L.crossing) {
+        double t;
+        if (disk_root_mode == 1) {
+            t = disk_root_s(s.y, ns.y, k1.y, k7.y, d.y, hs);
+        } else {
+            double den = s.y - ns.y;
+            t = den != 0.0 ? s.y / den : 0.5;
+            t = fmin(fmax(t, 0.0), 1.0);
+        }
+        L.hx = dense_s(s.x, ns.x, k1.x, k7.x, d.x, hs, t);
+        L.hz = dense_s(s.z, ns.z, k1.z, k7.z, d.z, hs, t);
+        L.hpx = dense_s(s.px, ns.px, k1.px, k7.px, d.px, hs, t);
+        L.hpy = dense_s(s.py, ns.py, k1.py, k7.py, d.py, hs, t);
+        L.hpz = dense_s(s.pz, ns.pz, k1.pz, k7.pz, d.pz, hs, t);
+    }
     L.post = true;
 }
 
