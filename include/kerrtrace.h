@@ -189,7 +189,9 @@ typedef struct kt_launch_opts {
     int32_t waves_per_cu;   /* resident waves per CU of the persistent grid */
     int32_t refill_threshold; /* refill a wave once this many lanes are idle */
     int32_t queue_chunk;    /* pixel tile edge used to order the work queue */
-    int32_t reserved;
+    int32_t cost_order;     /* 0 or 1: visit tiles hardest-first (predicted
+                               from the rays' closeness to the photon-orbit
+                               critical curve); 2: row-major order */
 } kt_launch_opts;
 
 typedef struct kt_renderer kt_renderer;

@@ -134,7 +134,7 @@ class KtLaunchOpts(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("refill_threshold", C.c_int32),
         ("queue_chunk", C.c_int32),
-        ("reserved", C.c_int32),
+        ("cost_order", C.c_int32),
     ]
 
 

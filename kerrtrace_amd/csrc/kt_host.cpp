@@ -36,6 +36,8 @@ hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, c
                           int enabled, hipStream_t stream);
 hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int TH,
                          const float* d_wts, float intensity, int bloom_on, hipStream_t stream);
+hipError_t launch_cost_order(const RenderParams* d_params, int n_tiles, uint8_t* d_keys, uint32_t* d_hist,
+                             uint32_t* d_perm, hipStream_t stream);
 size_t bloom_h_lds_bytes(int R);
 size_t post_v_lds_bytes(int R, int TH);
 }  // namespace kt
@@ -235,6 +237,11 @@ struct kt_renderer {
     int post_th = 64;
     kt::DevCounters* d_counters = nullptr;
     uint32_t* d_trace_flag = nullptr;
+    // hardest-first tile order: keys, bucket counts/offsets, permutation
+    uint8_t* d_cost_keys = nullptr;
+    uint32_t* d_cost_hist = nullptr;
+    uint32_t* d_tile_perm = nullptr;
+    size_t max_tiles = 0;
     // ring of parameter records: pinned host staging -> device
     static constexpr int kRing = 16;
     kt::RenderParams* h_params = nullptr;
@@ -242,7 +249,7 @@ struct kt_renderer {
     hipEvent_t param_ev[kRing] = {};
     bool param_ev_used[kRing] = {};
     int param_next = 0;
-    kt_launch_opts opts{8, 8, 8, 0};
+    kt_launch_opts opts{8, 8, 8, 1};
     hipStream_t last_stream = nullptr;
     uint64_t last_rays = 0;
     // double-buffered frames (mod.rs:24-51, 88-91)
@@ -302,6 +309,9 @@ void release(kt_renderer* r) {
     if (r->d_bloom_w) (void)hipFree(r->d_bloom_w);
     if (r->d_counters) (void)hipFree(r->d_counters);
     if (r->d_trace_flag) (void)hipFree(r->d_trace_flag);
+    if (r->d_cost_keys) (void)hipFree(r->d_cost_keys);
+    if (r->d_cost_hist) (void)hipFree(r->d_cost_hist);
+    if (r->d_tile_perm) (void)hipFree(r->d_tile_perm);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
 }
@@ -387,9 +397,16 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     kt::RenderParams P;
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
     make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
+    const bool order = r->opts.cost_order != 2 && tile <= 8;
+    const int n_tiles = P.tiles_x * (((int)r->cfg.height + tile - 1) / tile);
+    if (order) P.tile_perm = r->d_tile_perm;
     kt::RenderParams* dp = nullptr;
     int rc = upload_params(r, P, s, &dp);
     if (rc != KT_OK) return rc;
+    if (order) {
+        hipError_t e = kt::launch_cost_order(dp, n_tiles, r->d_cost_keys, r->d_cost_hist, r->d_tile_perm, s);
+        if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("cost order launch: ") + hipGetErrorString(e));
+    }
     KT_HIP(r, kt::launch_reset(r->d_counters, flag, s), "counter reset launch");
     int wpc = r->opts.waves_per_cu > 0 ? r->opts.waves_per_cu : 8;
     int blocks = std::max(1, (r->cus * wpc + 3) / 4);
@@ -603,6 +620,10 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     KT_CREATE_HIP(hipMalloc(&r->d_counters, sizeof(kt::DevCounters)), "hipMalloc counters");
     KT_CREATE_HIP(hipMemset(r->d_counters, 0, sizeof(kt::DevCounters)), "clear counters");
     KT_CREATE_HIP(hipMalloc(&r->d_trace_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
+    r->max_tiles = (size_t)cfg->width * cfg->height;  // tile edge 1 is the largest count
+    KT_CREATE_HIP(hipMalloc(&r->d_cost_keys, r->max_tiles), "hipMalloc cost keys");
+    KT_CREATE_HIP(hipMalloc(&r->d_cost_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_CREATE_HIP(hipMalloc(&r->d_tile_perm, r->max_tiles * sizeof(uint32_t)), "hipMalloc tile order");
     KT_CREATE_HIP(hipMemset(r->d_trace_flag, 0, 2 * sizeof(uint32_t)), "clear flag");
     KT_CREATE_HIP(hipHostMalloc(&r->h_params, sizeof(kt::RenderParams) * kt_renderer::kRing, hipHostMallocDefault), "pinned params");
     KT_CREATE_HIP(hipMalloc(&r->d_params, sizeof(kt::RenderParams) * kt_renderer::kRing), "hipMalloc params");
@@ -648,7 +669,7 @@ kt_status kt_get_luts(const kt_renderer* r, double* disk, double* bb) {
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* o) {
     if (!r || !o) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
-        o->queue_chunk < 0 || o->queue_chunk > 256)
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->cost_order < 0 || o->cost_order > 2)
         return set_err(r, KT_ERR_INVALID_ARGUMENT, "launch option out of range");
     r->opts = *o;
     return KT_OK;

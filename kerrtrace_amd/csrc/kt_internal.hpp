@@ -42,6 +42,7 @@ struct RenderParams {
     uint64_t ray_count;       // work items of this render: W * H
     int32_t tile;             // work-queue tile edge (pixels)
     int32_t tiles_x;
+    const uint32_t* tile_perm;  // visiting order of the tiles, or null (row-major)
 };
 
 // Work counters, summed over the launch (one atomic per wave at exit).

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     K32 k1m{0, 0, 0, 0, 0, 0};  // FSAL stage-1 pieces of the current state
     LaneCounters cnt;
     bool drained = false;
+    bool high_prio = false;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
     if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
@@ -111,6 +112,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         }
         refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
         if (drained && __ballot(L.active) == 0) break;
+        // a wave holding a long-running ray (photon-ring orbits) issues
+        // ahead of the other wave(s) on its SIMD, so its serial chain of
+        // steps, which bounds the frame time, is not slowed by them
+        if (__ballot(L.active && L.ray_att > kLongRay) != 0) {
+            if (!high_prio) {
+                __builtin_amdgcn_s_setprio(2);
+                high_prio = true;
+            }
+        } else if (high_prio) {
+            __builtin_amdgcn_s_setprio(0);
+            high_prio = false;
+        }
         if (!L.active || L.finished) continue;
 
         // ---- one DP5(4) attempt, split form (see the file comment)

@@ -40,6 +40,8 @@ namespace kt {
 #define KT_E7 (-1.0 / 40.0)
 
 static constexpr int kWave = 64;
+// attempts after which a ray counts as long-running (the frame mean is ~40-80)
+static constexpr uint32_t kLongRay = 256;
 
 // Potential-scaled tolerance of the step that starts at s, whose
 // Kerr-Schild radius is r_c (cuda/kerr_trace.cuh:67-73, with the tolerance
@@ -251,8 +253,9 @@ __device__ __forceinline__ bool decode_item(const RenderParams* __restrict__ Ps,
     if (local >= P.ray_count) return false;
     uint32_t T = (uint32_t)P.tile;
     uint32_t tt = T * T;
-    uint32_t tile_id = (uint32_t)(local / tt);
-    uint32_t within = (uint32_t)(local - (uint64_t)tile_id * tt);
+    uint32_t slot = (uint32_t)(local / tt);
+    uint32_t within = (uint32_t)(local - (uint64_t)slot * tt);
+    uint32_t tile_id = P.tile_perm ? P.tile_perm[slot] : slot;
     uint32_t ty = tile_id / (uint32_t)P.tiles_x;
     uint32_t tx = tile_id - ty * (uint32_t)P.tiles_x;
     px = (int)(tx * T + within % T);

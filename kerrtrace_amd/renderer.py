@@ -67,8 +67,8 @@ class HipRenderer:
         return self.cfg.height
 
     # ---- stream-ordered device API
-    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0):
-        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, 0)
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0):
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order)
         check(lib().kt_set_launch_opts(self._h, C.byref(o)), self._h)
 
     def alloc_hdr(self):
