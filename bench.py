@@ -87,7 +87,7 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
     torch.cuda.synchronize()
     ref_hdr, ref_cls = hdr64.cpu().numpy().astype(np.float64), cls64.cpu().numpy()
     out = {}
-    for tol in (cfg.tolerance, 1e-7):
+    for tol in (cfg.tolerance, 1e-8, 1e-7):
         c = cfg.copy()
         c.precision = K.KT_PRECISION_MIXED
         c.tolerance = tol
@@ -102,7 +102,7 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
         torch.cuda.synchronize()
         m = compare(hm.cpu().numpy(), clm.cpu().numpy(), ref_hdr, ref_cls)
         rays = c.width * c.height * c.spp
-        key = "same_tol" if tol == cfg.tolerance else "tol_1e-7"
+        key = "same_tol" if tol == cfg.tolerance else f"tol_{tol:g}"
         out[key] = {
             "tolerance": tol,
             "trace_ms": round(t * 1e3, 4),
