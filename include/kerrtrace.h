@@ -182,6 +182,7 @@ typedef struct kt_trace_stats {
     float overflow_temperature; /* the first such temperature (kernel.cu:5-8) */
     double kernel_us;       /* first wave start to last wave exit (device clock) */
     double tail_us;         /* work queue empty to last wave exit */
+    double kernel_event_ms; /* HIP events recorded around the tracer launch alone */
 } kt_trace_stats;
 
 /* Tuning knobs of the persistent tracer (new).  Zero means "default". */

@@ -126,6 +126,7 @@ class KtTraceStats(C.Structure):
         ("overflow_temperature", C.c_float),
         ("kernel_us", C.c_double),
         ("tail_us", C.c_double),
+        ("kernel_event_ms", C.c_double),
     ]
 
 

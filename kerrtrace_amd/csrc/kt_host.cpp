@@ -252,6 +252,7 @@ struct kt_renderer {
     kt_launch_opts opts{8, 8, 8, 1};
     hipStream_t last_stream = nullptr;
     uint64_t last_rays = 0;
+    hipEvent_t ev_trace0 = nullptr, ev_trace1 = nullptr;  // around the tracer launch
     // double-buffered frames (mod.rs:24-51, 88-91)
     struct Frame {
         float* hdr = nullptr;
@@ -300,6 +301,8 @@ void release(kt_renderer* r) {
     }
     for (int i = 0; i < kt_renderer::kRing; ++i)
         if (r->param_ev[i]) (void)hipEventDestroy(r->param_ev[i]);
+    if (r->ev_trace0) (void)hipEventDestroy(r->ev_trace0);
+    if (r->ev_trace1) (void)hipEventDestroy(r->ev_trace1);
     if (r->h_params) (void)hipHostFree(r->h_params);
     if (r->d_params) (void)hipFree(r->d_params);
     if (r->d_disk_lut) (void)hipFree(r->d_disk_lut);
@@ -411,10 +414,12 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     int wpc = r->opts.waves_per_cu > 0 ? r->opts.waves_per_cu : 8;
     int blocks = std::max(1, (r->cus * wpc + 3) / 4);
     int thr = r->opts.refill_threshold > 0 ? std::min(r->opts.refill_threshold, 64) : 8;
+    KT_HIP(r, hipEventRecord(r->ev_trace0, s), "trace event");
     hipError_t e = r->cfg.precision == KT_PRECISION_MIXED
                        ? kt::launch_trace_mixed(dp, 1, P.ray_count, thr, r->d_counters, blocks, s)
                        : kt::launch_trace_fp64(dp, 1, P.ray_count, thr, r->d_counters, blocks, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
+    KT_HIP(r, hipEventRecord(r->ev_trace1, s), "trace event");
     r->last_stream = s;
     r->last_rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
     return KT_OK;
@@ -629,6 +634,8 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     KT_CREATE_HIP(hipMalloc(&r->d_params, sizeof(kt::RenderParams) * kt_renderer::kRing), "hipMalloc params");
     for (int i = 0; i < kt_renderer::kRing; ++i)
         KT_CREATE_HIP(hipEventCreateWithFlags(&r->param_ev[i], hipEventDisableTiming), "param event");
+    KT_CREATE_HIP(hipEventCreate(&r->ev_trace0), "trace event");
+    KT_CREATE_HIP(hipEventCreate(&r->ev_trace1), "trace event");
     // frame buffers (init.rs:142-178), allocated for both frames
     const size_t px = (size_t)cfg->width * cfg->height;
     for (auto& f : r->frames) {
@@ -712,6 +719,8 @@ kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
     out->tail_us = (c.clock_drain != ~0ull && c.clock_end > c.clock_drain)
                        ? (double)(c.clock_end - c.clock_drain) * us_per_tick
                        : 0.0;
+    float ms = 0.0f;
+    out->kernel_event_ms = hipEventElapsedTime(&ms, r->ev_trace0, r->ev_trace1) == hipSuccess ? (double)ms : 0.0;
     return KT_OK;
 }
 
