@@ -252,6 +252,22 @@ kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom,
  * disk: double[lut_size]; blackbody: double[lut_size*3] (rgb). */
 kt_status kt_get_luts(const kt_renderer* r, double* disk_lut, double* bb_lut);
 
+/* ---- batched persistent launch (new: the reference renders one frame per
+ * launch) --------------------------------------------------------------
+ * n renders with their own configs and cameras (spin, inclination,
+ * resolution, ...; one precision mode for all) traced by one persistent
+ * launch whose work queue holds the tiles of every render, hardest first.
+ * hdr[i]: float4[W_i*H_i]; hit_class: NULL or an array of n pointers
+ * (entries may be NULL).  Errors are read with kt_last_error(NULL, ...). */
+typedef struct kt_batch kt_batch;
+kt_status kt_batch_create(const kt_config* cfgs, const kt_camera* cams,
+                          int32_t n, kt_batch** out);
+void kt_batch_destroy(kt_batch* b);
+kt_status kt_batch_set_launch_opts(kt_batch* b, const kt_launch_opts* opts);
+kt_status kt_batch_trace(kt_batch* b, float* const* hdr,
+                         uint8_t* const* hit_class, void* stream);
+kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,9 @@ accretion-rate scale, tolerance floor, observer placement).
 Presets name the five workloads the renderer is measured and checked on,
 by what they are.
 """
+import json
 import math
+import os
 
 import yaml
 
@@ -326,3 +328,22 @@ def scaled(cfg, cam, width, height):
     c = cfg.copy()
     c.width, c.height = int(width), int(height)
     return c, cam.copy()
+
+
+BATCH64_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "presets", "batch64_seed7.json")
+
+
+def batch_preset(path=BATCH64_PATH, limit=None, max_side=None):
+    """The 64-render batch (seed 7 draw, committed JSON): a list of
+    (KtConfig, KtCamera).  Knobs the draw does not set follow hd1080_a0998
+    (FOV 20 deg, tol 1e-9, disk ISCO..30 M, 6 octaves seed 1, sigma 12 px).
+    limit keeps the first renders; max_side caps a render's side (the scene
+    is unchanged, only the pixel count)."""
+    with open(path, "r", encoding="utf-8") as fh:
+        doc = json.load(fh)
+    out = []
+    for r in doc["renders"][:limit]:
+        side = int(r["side"]) if max_side is None else min(int(r["side"]), int(max_side))
+        out.append(_observer_preset(side, side, float(r["spin"]), float(r["inclination_deg"]), float(r["r_obs"]),
+                                    20.0, 1e-9, 30.0, 6, 1, 12.0))
+    return out

@@ -36,8 +36,8 @@ hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, c
                           int enabled, hipStream_t stream);
 hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int TH,
                          const float* d_wts, float intensity, int bloom_on, hipStream_t stream);
-hipError_t launch_cost_order(const RenderParams* d_params, int n_tiles, uint8_t* d_keys, uint32_t* d_hist,
-                             uint32_t* d_perm, hipStream_t stream);
+hipError_t launch_cost_order(const RenderParams* d_params, int n_renders, int n_tiles, uint8_t* d_keys,
+                             uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream);
 size_t bloom_h_lds_bytes(int R);
 size_t post_v_lds_bytes(int R, int TH);
 }  // namespace kt
@@ -377,8 +377,8 @@ void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, con
     P.tile = tile;
     P.tiles_x = ((int)c.width + tile - 1) / tile;
     int tiles_y = ((int)c.height + tile - 1) / tile;
-    P.ray_begin = 0;
-    P.ray_count = (uint64_t)P.tiles_x * (uint64_t)tiles_y * (uint64_t)tile * (uint64_t)tile;
+    P.tile_begin = 0;
+    P.n_tiles = (uint32_t)(P.tiles_x * tiles_y);
 }
 
 // Stages a parameter record through the pinned ring and returns its
@@ -401,13 +401,14 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
     make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
     const bool order = r->opts.cost_order != 2 && tile <= 8;
-    const int n_tiles = P.tiles_x * (((int)r->cfg.height + tile - 1) / tile);
+    const int n_tiles = (int)P.n_tiles;
+    const uint64_t items = (uint64_t)n_tiles * (uint64_t)tile * (uint64_t)tile;
     if (order) P.tile_perm = r->d_tile_perm;
     kt::RenderParams* dp = nullptr;
     int rc = upload_params(r, P, s, &dp);
     if (rc != KT_OK) return rc;
     if (order) {
-        hipError_t e = kt::launch_cost_order(dp, n_tiles, r->d_cost_keys, r->d_cost_hist, r->d_tile_perm, s);
+        hipError_t e = kt::launch_cost_order(dp, 1, n_tiles, r->d_cost_keys, r->d_cost_hist, r->d_tile_perm, s);
         if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("cost order launch: ") + hipGetErrorString(e));
     }
     KT_HIP(r, kt::launch_reset(r->d_counters, flag, s), "counter reset launch");
@@ -416,8 +417,8 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     int thr = r->opts.refill_threshold > 0 ? std::min(r->opts.refill_threshold, 64) : 8;
     KT_HIP(r, hipEventRecord(r->ev_trace0, s), "trace event");
     hipError_t e = r->cfg.precision == KT_PRECISION_MIXED
-                       ? kt::launch_trace_mixed(dp, 1, P.ray_count, thr, r->d_counters, blocks, s)
-                       : kt::launch_trace_fp64(dp, 1, P.ray_count, thr, r->d_counters, blocks, s);
+                       ? kt::launch_trace_mixed(dp, 1, items, thr, r->d_counters, blocks, s)
+                       : kt::launch_trace_fp64(dp, 1, items, thr, r->d_counters, blocks, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
     KT_HIP(r, hipEventRecord(r->ev_trace1, s), "trace event");
     r->last_stream = s;
@@ -437,6 +438,62 @@ int post_into(kt_renderer* r, const float* hdr, float* bloom, uint32_t* ldr, hip
                                      on ? 1 : 0, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("post launch: ") + hipGetErrorString(e));
     return KT_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ batch launch
+// Many renders (different spins, cameras, resolutions) traced by one
+// persistent launch: one parameter record per render, their tiles numbered
+// consecutively in one work queue (hardest-first across all renders).
+struct kt_batch {
+    int n = 0;
+    std::vector<kt_config> cfgs;
+    std::vector<kt_camera> cams;
+    std::vector<Derived> ds;
+    std::string err;
+    int device = 0;
+    int cus = 256;
+    int precision = KT_PRECISION_FP64;
+    std::vector<double*> d_disk_luts;
+    std::vector<double*> d_bb_luts;
+    kt::RenderParams* h_params = nullptr;  // pinned
+    kt::RenderParams* d_params = nullptr;
+    kt::DevCounters* d_counters = nullptr;
+    uint32_t* d_flag = nullptr;
+    uint8_t* d_keys = nullptr;
+    uint32_t* d_hist = nullptr;
+    uint32_t* d_perm = nullptr;
+    size_t max_tiles = 0;
+    kt_launch_opts opts{8, 8, 8, 1};
+    hipStream_t last_stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_params = nullptr;
+    bool params_in_flight = false;
+    uint64_t last_rays = 0;
+};
+
+namespace {
+
+void release_batch(kt_batch* b) {
+    if (!b) return;
+    for (double* p : b->d_disk_luts) if (p) (void)hipFree(p);
+    for (double* p : b->d_bb_luts) if (p) (void)hipFree(p);
+    if (b->h_params) (void)hipHostFree(b->h_params);
+    if (b->d_params) (void)hipFree(b->d_params);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->d_flag) (void)hipFree(b->d_flag);
+    if (b->d_keys) (void)hipFree(b->d_keys);
+    if (b->d_hist) (void)hipFree(b->d_hist);
+    if (b->d_perm) (void)hipFree(b->d_perm);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev_params) (void)hipEventDestroy(b->ev_params);
+    delete b;
+}
+
+bool same_blackbody(const kt_config& a, const kt_config& b) {
+    return a.lut_size == b.lut_size && a.lut_max_temp == b.lut_max_temp && a.wavelength_start == b.wavelength_start &&
+           a.wavelength_end == b.wavelength_end && a.wavelength_step == b.wavelength_step;
 }
 
 }  // namespace
@@ -810,6 +867,182 @@ kt_status kt_finish_frame(kt_renderer* r, int32_t index) {
     }
     r->queue.pop_front();
     r->frames[index].pending = false;
+    return KT_OK;
+}
+
+
+kt_status kt_batch_create(const kt_config* cfgs, const kt_camera* cams, int32_t n, kt_batch** out) {
+    if (!cfgs || !cams || !out || n <= 0) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument or n <= 0");
+    *out = nullptr;
+    if (n > 65536) return set_err(nullptr, KT_ERR_CAPACITY, "more than 65536 renders in one batch");
+    kt_batch* b = new (std::nothrow) kt_batch();
+    if (!b) return set_err(nullptr, KT_ERR_DEVICE, "out of host memory");
+    b->n = n;
+    b->cfgs.assign(cfgs, cfgs + n);
+    b->cams.assign(cams, cams + n);
+    b->ds.resize(n);
+    b->precision = cfgs[0].precision;
+    uint64_t px = 0;
+    for (int i = 0; i < n; ++i) {
+        std::string why;
+        int rc = derive(b->cfgs[i], b->ds[i], why);
+        if (rc != KT_OK) {
+            release_batch(b);
+            return set_err(nullptr, rc, "render " + std::to_string(i) + ": " + why);
+        }
+        if (cfgs[i].precision != b->precision) {
+            release_batch(b);
+            return set_err(nullptr, KT_ERR_INVALID_CONFIG, "all renders of a batch must share the precision mode");
+        }
+        px += (uint64_t)cfgs[i].width * cfgs[i].height;
+    }
+#define KT_BATCH_HIP(call, what)                  \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess) {                   \
+            int rc_ = hip_err(nullptr, e_, what); \
+            release_batch(b);                     \
+            return rc_;                           \
+        }                                         \
+    } while (0)
+    KT_BATCH_HIP(hipGetDevice(&b->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    KT_BATCH_HIP(hipGetDeviceProperties(&prop, b->device), "hipGetDeviceProperties");
+    b->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    b->d_disk_luts.assign(n, nullptr);
+    b->d_bb_luts.assign(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+        const size_t m = cfgs[i].lut_size;
+        KT_BATCH_HIP(hipMalloc(&b->d_disk_luts[i], m * sizeof(double)), "hipMalloc disk LUT");
+        KT_BATCH_HIP(hipMemcpy(b->d_disk_luts[i], b->ds[i].disk_lut.data(), m * sizeof(double), hipMemcpyHostToDevice),
+                     "upload disk LUT");
+        // renders with the same blackbody table share one device copy
+        int same = -1;
+        for (int j = 0; j < i && same < 0; ++j)
+            if (same_blackbody(cfgs[i], cfgs[j])) same = j;
+        if (same >= 0) continue;
+        KT_BATCH_HIP(hipMalloc(&b->d_bb_luts[i], m * 3 * sizeof(double)), "hipMalloc blackbody LUT");
+        KT_BATCH_HIP(hipMemcpy(b->d_bb_luts[i], b->ds[i].bb_lut.data(), m * 3 * sizeof(double), hipMemcpyHostToDevice),
+                     "upload blackbody LUT");
+    }
+    KT_BATCH_HIP(hipHostMalloc(&b->h_params, sizeof(kt::RenderParams) * n, hipHostMallocDefault), "pinned params");
+    KT_BATCH_HIP(hipMalloc(&b->d_params, sizeof(kt::RenderParams) * n), "hipMalloc params");
+    KT_BATCH_HIP(hipMalloc(&b->d_counters, sizeof(kt::DevCounters)), "hipMalloc counters");
+    KT_BATCH_HIP(hipMalloc(&b->d_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
+    KT_BATCH_HIP(hipMemset(b->d_flag, 0, 2 * sizeof(uint32_t)), "clear flag");
+    // tile order buffers for tile edges >= 2
+    size_t tiles = 0;
+    for (int i = 0; i < n; ++i) tiles += (size_t)((cfgs[i].width + 1) / 2) * ((cfgs[i].height + 1) / 2);
+    b->max_tiles = tiles;
+    KT_BATCH_HIP(hipMalloc(&b->d_keys, tiles), "hipMalloc cost keys");
+    KT_BATCH_HIP(hipMalloc(&b->d_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_BATCH_HIP(hipMalloc(&b->d_perm, tiles * sizeof(uint32_t)), "hipMalloc tile order");
+    KT_BATCH_HIP(hipEventCreate(&b->ev0), "event");
+    KT_BATCH_HIP(hipEventCreate(&b->ev1), "event");
+    KT_BATCH_HIP(hipEventCreateWithFlags(&b->ev_params, hipEventDisableTiming), "event");
+#undef KT_BATCH_HIP
+    (void)px;
+    *out = b;
+    return KT_OK;
+}
+
+void kt_batch_destroy(kt_batch* b) {
+    if (!b) return;
+    if (b->last_stream) (void)hipStreamSynchronize(b->last_stream);
+    release_batch(b);
+}
+
+kt_status kt_batch_set_launch_opts(kt_batch* b, const kt_launch_opts* o) {
+    if (!b || !o) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->queue_chunk == 1 || o->cost_order < 0 || o->cost_order > 2)
+        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "launch option out of range (batch tile edge >= 2)");
+    b->opts = *o;
+    return KT_OK;
+}
+
+kt_status kt_batch_trace(kt_batch* b, float* const* hdr, uint8_t* const* cls, void* stream) {
+    if (!b || !hdr) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int tile = b->opts.queue_chunk > 0 ? b->opts.queue_chunk : 8;
+    // the pinned records of the previous call must have reached the device
+    if (b->params_in_flight) {
+        hipError_t e = hipEventSynchronize(b->ev_params);
+        if (e != hipSuccess) return hip_err(nullptr, e, "param upload wait");
+    }
+    uint32_t tile_begin = 0;
+    uint64_t rays = 0;
+    for (int i = 0; i < b->n; ++i) {
+        if (!hdr[i]) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null HDR pointer");
+        kt::RenderParams& P = b->h_params[i];
+        make_params(nullptr, b->cfgs[i], b->ds[i], b->cams[i], hdr[i], cls ? cls[i] : nullptr, b->d_flag, tile, P);
+        P.disk_lut = b->d_disk_luts[i];
+        int bb = i;
+        while (!b->d_bb_luts[bb]) --bb;  // the first render with the same table owns it
+        P.bb_lut = b->d_bb_luts[bb];
+        P.tile_begin = tile_begin;
+        tile_begin += P.n_tiles;
+        rays += (uint64_t)b->cfgs[i].width * b->cfgs[i].height * b->cfgs[i].spp;
+    }
+    const bool order = b->opts.cost_order != 2 && tile <= 8 && tile_begin <= b->max_tiles;
+    for (int i = 0; i < b->n; ++i) b->h_params[i].tile_perm = order ? b->d_perm : nullptr;
+    hipError_t e = hipMemcpyAsync(b->d_params, b->h_params, sizeof(kt::RenderParams) * b->n, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "param upload");
+    e = hipEventRecord(b->ev_params, s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "param event");
+    b->params_in_flight = true;
+    e = kt::launch_reset(b->d_counters, b->d_flag, s);
+    if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("reset launch: ") + hipGetErrorString(e));
+    if (order) {
+        e = kt::launch_cost_order(b->d_params, b->n, (int)tile_begin, b->d_keys, b->d_hist, b->d_perm, s);
+        if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("cost order: ") + hipGetErrorString(e));
+    }
+    const uint64_t items = (uint64_t)tile_begin * (uint64_t)tile * (uint64_t)tile;
+    int wpc = b->opts.waves_per_cu > 0 ? b->opts.waves_per_cu : 8;
+    int blocks = std::max(1, (b->cus * wpc + 3) / 4);
+    int thr = b->opts.refill_threshold > 0 ? std::min(b->opts.refill_threshold, 64) : 8;
+    e = hipEventRecord(b->ev0, s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "trace event");
+    e = b->precision == KT_PRECISION_MIXED ? kt::launch_trace_mixed(b->d_params, b->n, items, thr, b->d_counters, blocks, s)
+                                           : kt::launch_trace_fp64(b->d_params, b->n, items, thr, b->d_counters, blocks, s);
+    if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("batch trace launch: ") + hipGetErrorString(e));
+    e = hipEventRecord(b->ev1, s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "trace event");
+    b->last_stream = s;
+    b->last_rays = rays;
+    return KT_OK;
+}
+
+kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out) {
+    if (!b || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipStreamSynchronize(b->last_stream);
+    if (e != hipSuccess) return hip_err(nullptr, e, "stream sync");
+    kt::DevCounters c;
+    uint32_t flag[2];
+    e = hipMemcpy(&c, b->d_counters, sizeof c, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(flag, b->d_flag, sizeof flag, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_err(nullptr, e, "read counters");
+    std::memset(out, 0, sizeof *out);
+    out->rays = b->last_rays;
+    out->rhs_evals = c.rhs_evals;
+    out->attempts = c.attempts;
+    out->accepted = c.accepted;
+    out->crossings = c.crossings;
+    out->max_attempts_ray = c.max_attempts_ray;
+    out->lut_overflow = flag[0];
+    float T;
+    std::memcpy(&T, &flag[1], sizeof T);
+    out->overflow_temperature = flag[0] ? T : 0.0f;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, b->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    const double us_per_tick = 1000.0 / (double)rate_khz;
+    out->kernel_us = c.clock_end > c.clock_start ? (double)(c.clock_end - c.clock_start) * us_per_tick : 0.0;
+    out->tail_us = (c.clock_drain != ~0ull && c.clock_end > c.clock_drain)
+                       ? (double)(c.clock_end - c.clock_drain) * us_per_tick
+                       : 0.0;
+    float ms = 0.0f;
+    out->kernel_event_ms = hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess ? (double)ms : 0.0;
     return KT_OK;
 }
 

@@ -38,11 +38,13 @@ struct RenderParams {
     int32_t W, H, spp, max_steps, max_attempts, lut_n, noise_detail, disk_root_mode;
     int32_t rhs_exact;        // 1: exact Hamiltonian dr^2/dy
     int32_t reserved;
-    uint64_t ray_begin;       // first global work item of this render (batched launches)
-    uint64_t ray_count;       // work items of this render: W * H
-    int32_t tile;             // work-queue tile edge (pixels)
+    // work queue: items are (global tile, pixel within the tile); the
+    // tiles of all renders of a launch are numbered consecutively
+    uint32_t tile_begin;      // first global tile of this render
+    uint32_t n_tiles;         // tiles of this render: tiles_x * tiles_y
+    int32_t tile;             // tile edge in pixels (the same for every render of a launch)
     int32_t tiles_x;
-    const uint32_t* tile_perm;  // visiting order of the tiles, or null (row-major)
+    const uint32_t* tile_perm;  // visiting order of the launch's global tiles, or null
 };
 
 // Work counters, summed over the launch (one atomic per wave at exit).

@@ -26,14 +26,21 @@ namespace kt {
 
 static constexpr int kBuckets = 256;
 
-// One wave per tile (tile edge T <= 8: T*T <= 64 lanes); key = max over the
-// tile's pixels, quantised to 8 bits.
-__global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict__ Ps, uint8_t* __restrict__ keys,
-                                                 int n_tiles) {
-    const RenderParams& P = Ps[0];
+// One wave per global tile (tile edge T <= 8: T*T <= 64 lanes); key = max
+// over the tile's pixels, quantised to 8 bits.  Renders are sorted by
+// tile_begin; a binary search finds a tile's render.
+__global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict__ Ps, int n_renders,
+                                                 uint8_t* __restrict__ keys, int n_tiles) {
     const int lane = threadIdx.x & 63;
-    const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    if (tile >= n_tiles) return;
+    const int gtile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (gtile >= n_tiles) return;
+    int lo = 0, hi = n_renders - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((int)Ps[mid].tile_begin <= gtile) lo = mid; else hi = mid - 1;
+    }
+    const RenderParams& P = Ps[lo];
+    const int tile = gtile - (int)P.tile_begin;
     const int T = P.tile;
     const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
     const int px = tx * T + lane % T, py = ty * T + lane / T;
@@ -73,7 +80,7 @@ __global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict_
     if (lane == 0) {
         // -log spans about [-10, 28]; 6 levels per unit of log
         int q = (int)((key + 10.0f) * 6.0f);
-        keys[tile] = (uint8_t)min(max(q, 0), kBuckets - 1);
+        keys[gtile] = (uint8_t)min(max(q, 0), kBuckets - 1);
     }
 }
 
@@ -118,11 +125,12 @@ __global__ void zero_u32(uint32_t* p, int n) {
 
 // Builds perm (n_tiles entries): tile indices in descending predicted cost.
 // scratch: keys (n_tiles bytes, 4-byte aligned region) + 2 * 256 uint32.
-hipError_t launch_cost_order(const RenderParams* d_params, int n_tiles, uint8_t* d_keys, uint32_t* d_hist,
-                             uint32_t* d_perm, hipStream_t stream) {
+hipError_t launch_cost_order(const RenderParams* d_params, int n_renders, int n_tiles, uint8_t* d_keys,
+                             uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream) {
     uint32_t* d_off = d_hist + kBuckets;
     hipLaunchKernelGGL(zero_u32, dim3(2), dim3(256), 0, stream, d_hist, 2 * kBuckets);
-    hipLaunchKernelGGL(cost_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, d_params, d_keys, n_tiles);
+    hipLaunchKernelGGL(cost_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, d_params, n_renders, d_keys,
+                       n_tiles);
     int blocks = min(1024, (n_tiles + 255) / 256);
     hipLaunchKernelGGL(key_histogram, dim3(blocks), dim3(256), 0, stream, d_keys, n_tiles, d_hist);
     hipLaunchKernelGGL(key_scan, dim3(1), dim3(kBuckets), 0, stream, d_hist, d_off);

@@ -236,33 +236,40 @@ __device__ __forceinline__ const RenderParams& prm(const RenderParams* __restric
     return BATCH ? Ps[rid] : Ps[0];
 }
 
+// Render owning global tile g (records sorted by tile_begin).
+template <bool BATCH>
+__device__ __forceinline__ int render_of_tile(const RenderParams* __restrict__ Ps, int n_renders, uint32_t g) {
+    if (!BATCH) return 0;
+    int lo = 0, hi = n_renders - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (Ps[mid].tile_begin <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Decode a work item: item = slot * T^2 + within, where slot is the visiting
+// position of a global tile (tile_perm maps it to the tile; row-major
+// without it) and within the pixel inside the tile.  Pixels of edge tiles
+// that fall outside their image return false.
 template <bool BATCH>
 __device__ __forceinline__ bool decode_item(const RenderParams* __restrict__ Ps, int n_renders, uint64_t item,
                                             int& rid, int& px, int& py) {
-    rid = 0;
-    if (BATCH) {
-        int lo = 0, hi = n_renders - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (Ps[mid].ray_begin <= item) lo = mid; else hi = mid - 1;
-        }
-        rid = lo;
-    }
+    const uint32_t T = (uint32_t)Ps[0].tile;
+    const uint32_t tt = T * T;
+    const uint32_t slot = (uint32_t)(item / tt);
+    const uint32_t within = (uint32_t)(item - (uint64_t)slot * tt);
+    const uint32_t g = Ps[0].tile_perm ? Ps[0].tile_perm[slot] : slot;
+    rid = render_of_tile<BATCH>(Ps, n_renders, g);
     const RenderParams& P = prm<BATCH>(Ps, rid);
-    uint64_t local = item - P.ray_begin;
-    if (local >= P.ray_count) return false;
-    uint32_t T = (uint32_t)P.tile;
-    uint32_t tt = T * T;
-    uint32_t slot = (uint32_t)(local / tt);
-    uint32_t within = (uint32_t)(local - (uint64_t)slot * tt);
-    uint32_t tile_id = P.tile_perm ? P.tile_perm[slot] : slot;
-    uint32_t ty = tile_id / (uint32_t)P.tiles_x;
-    uint32_t tx = tile_id - ty * (uint32_t)P.tiles_x;
+    const uint32_t tile_id = g - P.tile_begin;
+    if (tile_id >= P.n_tiles) return false;
+    const uint32_t ty = tile_id / (uint32_t)P.tiles_x;
+    const uint32_t tx = tile_id - ty * (uint32_t)P.tiles_x;
     px = (int)(tx * T + within % T);
     py = (int)(ty * T + within / T);
     return px < P.W && py < P.H;
 }
-
 
 // Per-lane ray state of the persistent tracer.
 struct Lane {

@@ -154,6 +154,83 @@ class HipRenderer:
         check(lib().kt_finish_frame(self._h, int(index)), self._h)
 
 
+class HipBatch:
+    """Many renders traced by one persistent launch (kt_batch_*): each keeps
+    its own config and camera; their tiles share one hardest-first work
+    queue.  All renders use the same precision mode."""
+
+    def __init__(self, renders):
+        _torch()
+        self.cfgs = [c.copy() for c, _ in renders]
+        self.cams = [cam.copy() for _, cam in renders]
+        n = len(self.cfgs)
+        cfg_arr = (KtConfig * n)(*self.cfgs)
+        cam_arr = (KtCamera * n)(*self.cams)
+        h = C.c_void_p()
+        check(lib().kt_batch_create(cfg_arr, cam_arr, n, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    @property
+    def rays(self):
+        return sum(c.width * c.height * c.spp for c in self.cfgs)
+
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0):
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order)
+        check(lib().kt_batch_set_launch_opts(self._h, C.byref(o)))
+
+    def alloc(self, with_class=True):
+        torch = _torch()
+        hdr = [torch.empty((c.height, c.width, 4), dtype=torch.float32, device="cuda") for c in self.cfgs]
+        cls = [torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda") for c in self.cfgs] \
+            if with_class else None
+        return hdr, cls
+
+    def trace(self, hdr=None, cls=None, with_class=True, stream=None):
+        """Trace every render on the current torch stream; returns lists of
+        float32 [H, W, 4] and (optionally) uint8 [H, W] tensors."""
+        torch = _torch()
+        if hdr is None:
+            hdr, c2 = self.alloc(with_class and cls is None)
+            cls = cls if cls is not None else c2
+        n = len(self.cfgs)
+        if len(hdr) != n or (cls is not None and len(cls) != n):
+            raise ValueError(f"expected {n} output tensors per list")
+        for i, c in enumerate(self.cfgs):
+            _check_tensor(hdr[i], (c.height, c.width, 4), torch.float32)
+            if cls is not None:
+                _check_tensor(cls[i], (c.height, c.width), torch.uint8)
+        hp = (C.c_void_p * n)(*[t.data_ptr() for t in hdr])
+        cp = (C.c_void_p * n)(*[t.data_ptr() for t in cls]) if cls is not None else None
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().kt_batch_trace(self._h, hp, cp, C.c_void_p(s.cuda_stream)))
+        return hdr, cls
+
+    def stats(self):
+        st = KtTraceStats()
+        check(lib().kt_batch_get_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+
 def _check_tensor(t, shape, dtype):
     if tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
         raise ValueError(f"expected a contiguous cuda tensor {shape} {dtype}, got {tuple(t.shape)} {t.dtype}")
