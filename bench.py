@@ -121,6 +121,57 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
     return out
 
 
+def batch_leg(args):
+    """The 64-render seed-7 batch (kerrtrace_amd/presets/batch64_seed7.json):
+    one persistent launch over every render's tiles vs 64 launches of one
+    render each, both timed with HIP events around the tracer launches."""
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.renderer import HipBatch, HipRenderer
+    renders = K.batch_preset()
+    rays = sum(c.width * c.height * c.spp for c, _ in renders)
+    b = HipBatch(renders)
+    hdr, _ = b.alloc(with_class=False)
+    b.trace(hdr=hdr)
+    torch.cuda.synchronize()
+    t_batch = []
+    for _ in range(args.steps):
+        b.trace(hdr=hdr)
+        st = b.stats()
+        t_batch.append(st["kernel_event_ms"])
+    singles = [HipRenderer(c) for c, _ in renders]
+    for r, (_, cam), h in zip(singles, renders, hdr):
+        r.trace(cam, hdr=h, with_class=False)
+    torch.cuda.synchronize()
+    t_seq = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for r, (_, cam), h in zip(singles, renders, hdr):
+            r.trace(cam, hdr=h, with_class=False)
+        e1.record()
+        torch.cuda.synchronize()
+        t_seq.append(e0.elapsed_time(e1))
+    for r in singles:
+        r.close()
+    b.close()
+    tb = sum(t_batch) / len(t_batch)
+    ts = sum(t_seq) / len(t_seq)
+    return {
+        "renders": len(renders),
+        "rays": rays,
+        "batched_ms": round(tb, 4),
+        "batched_mrays_s": round(rays / tb / 1e3, 4),
+        "sequential_ms": round(ts, 4),
+        "sequential_mrays_s": round(rays / ts / 1e3, 4),
+        "speedup": round(ts / tb, 4),
+        "attempts_per_ray": round(st["attempts"] / rays, 3),
+        "max_attempts_ray": st["max_attempts_ray"],
+        "tail_ms": round(st["tail_us"] / 1e3, 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +180,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-render batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
@@ -216,6 +268,9 @@ def main():
     print(f"[bench] timed {args.steps} frames: {ms_per_step:.2f} ms/frame", file=sys.stderr, flush=True)
     if not args.no_mixed:
         out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_kernel)
+    if not args.no_batch:
+        print("[bench] batch leg", file=sys.stderr, flush=True)
+        out["batch64"] = batch_leg(args)
 
     if not args.no_cpu:
         print("[bench] cpu baseline leg", file=sys.stderr, flush=True)
