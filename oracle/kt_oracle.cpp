@@ -895,6 +895,12 @@ double kto_dense(const double c[8], double h, double t) {
 double kto_dense_dt(const double c[8], double h, double t) {
     return dense_dt(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], h, t);
 }
+// c = {y0, y1, k1.y, k3.y, k4.y, k5.y, k6.y, k7.y}: true-root mode crossing
+double kto_disk_root(const double c[8], double h) {
+    State<double> k[6] = {};
+    for (int i = 0; i < 6; ++i) k[i].y = c[2 + i];
+    return disk_root_true(k[0], k[1], k[2], k[3], k[4], k[5], c[0], c[1], h);
+}
 double kto_fbm(double u, double v, int octaves, uint64_t seed) { return fbm(u, v, octaves, seed); }
 double kto_gradient_noise(double u, double v, uint64_t seed) { return gradient_noise(u, v, seed); }
 

@@ -48,6 +48,7 @@ def lib():
         L.kto_ks_r.argtypes, L.kto_ks_r.restype = [d, d, d, d], d
         L.kto_dense.argtypes, L.kto_dense.restype = [pd, d, d], d
         L.kto_dense_dt.argtypes, L.kto_dense_dt.restype = [pd, d, d], d
+        L.kto_disk_root.argtypes, L.kto_disk_root.restype = [pd, d], d
         L.kto_fbm.argtypes, L.kto_fbm.restype = [d, d, i32, u64], d
         L.kto_gradient_noise.argtypes, L.kto_gradient_noise.restype = [d, d, u64], d
         L.kto_derive.argtypes = [C.POINTER(KtConfig), C.POINTER(KtDerived), pd, pd]
@@ -110,6 +111,12 @@ def dense(y0, y1, k1, k3, k4, k5, k6, k7, h, t, derivative=False):
     c = np.array([y0, y1, k1, k3, k4, k5, k6, k7], dtype=np.float64)
     f = lib().kto_dense_dt if derivative else lib().kto_dense
     return f(_dptr(c), h, t)
+
+
+def disk_root(y0, y1, k1, k3, k4, k5, k6, k7, h):
+    """True-root crossing parameter t in [0, 1] of the dense output's y."""
+    c = np.array([y0, y1, k1, k3, k4, k5, k6, k7], dtype=np.float64)
+    return lib().kto_disk_root(_dptr(c), h)
 
 
 def fbm(u, v, octaves, seed=0):

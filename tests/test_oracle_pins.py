@@ -263,6 +263,38 @@ def _no_disk_cfg(a=0.0, R=1000.0, tol=1e-10):
     return cfg
 
 
+def test_true_disk_root_matches_bisection():
+    """The safeguarded Newton root of the dense-output y(t) equals a plain
+    bisection of the same interpolant (monotone cases: one root)."""
+    rng = np.random.default_rng(5)
+    checked = 0
+    for _ in range(400):
+        h = float(rng.uniform(0.01, 2.0))
+        y0 = float(rng.uniform(0.01, 1.0))
+        y1 = -float(rng.uniform(0.01, 1.0))
+        slope = (y1 - y0) / h
+        ks = [slope * float(rng.uniform(0.7, 1.3)) for _ in range(6)]
+        f = lambda t: O.dense(y0, y1, *ks, h, t)  # noqa: E731
+        ts = np.linspace(0.0, 1.0, 401)
+        vals = np.array([f(t) for t in ts])
+        if np.count_nonzero(np.diff(np.sign(vals))) != 1:
+            continue  # not a single crossing
+        lo, hi = 0.0, 1.0
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if mid in (lo, hi):
+                break
+            if (f(mid) >= 0.0) == (f(lo) >= 0.0):
+                lo = mid
+            else:
+                hi = mid
+        t = O.disk_root(y0, y1, *ks, h)
+        assert 0.0 <= t <= 1.0
+        assert t == pytest.approx(lo, abs=1e-13)
+        checked += 1
+    assert checked > 300
+
+
 @pytest.mark.parametrize("b,expect", [(5.0, 1), (5.15, 1), (5.25, 2), (5.5, 2)])
 def test_schwarzschild_capture_threshold(b, expect):
     """Capture iff impact parameter b < sqrt(27) M = 5.196 M."""
