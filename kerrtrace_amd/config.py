@@ -17,9 +17,10 @@ import os
 
 import yaml
 
-from ._abi import (KT_DISK_ROOT_SECANT_REF, KT_DISK_ROOT_TRUE,
-                   KT_PRECISION_FP64, KT_PRECISION_MIXED, KT_RHS_HAMILTONIAN,
-                   KT_RHS_REFERENCE, KtCamera, KtConfig)
+from ._abi import (KT_DISK_ROOT_SECANT_REF, KT_DISK_ROOT_TRUE,  # noqa: F401
+                   KT_END_ESCAPE, KT_END_HORIZON, KT_END_OTHER, KT_FLAG_DISK,
+                   KT_FLAG_SKY_GRID, KT_PRECISION_FP64, KT_PRECISION_MIXED,
+                   KT_RHS_HAMILTONIAN, KT_RHS_REFERENCE, KtCamera, KtConfig)
 
 U32_MAX = 0xFFFFFFFF
 
@@ -288,6 +289,14 @@ def _observer_preset(width, height, spin, incl, r_obs, fov, tol, r_out,
     return c, observer_camera(r_obs, incl, fov)
 
 
+# The differentiable scene: theta = (spin, inclination [rad], r_obs [M],
+# accretion-rate scale); the fit starts at GRAD_START and the target frame
+# is rendered at GRAD_TARGET.
+GRAD_FOV_DEG = 30.0
+GRAD_TARGET = (0.7, math.radians(70.0), 60.0, 1.0)
+GRAD_START = (0.5, math.radians(55.0), 70.0, 0.8)
+
+
 def preset(name, **overrides):
     """Return (KtConfig, KtCamera) of a named workload.
 
@@ -298,8 +307,8 @@ def preset(name, **overrides):
     uhd2160_i60     3840x2160, i=60 deg, tol 1e-7, mixed precision, other
                     knobs as hd1080_a0998 (pass spin=...)
     grad512         512x512 differentiable scene, a=0.7, i=70 deg, r=60M,
-                    mdot 1.0, tol 1e-7, mixed precision, other knobs as
-                    cpu256_a09
+                    mdot 1.0, FOV 30 deg, tol 1e-7, FP64, other knobs as
+                    cpu256_a09 (theta = GRAD_TARGET)
     """
     if name == "cpu256_a09":
         c, cam = _observer_preset(256, 256, 0.9, 80.0, 50.0, 30.0, 1e-8, 20.0, 4, 0, 8.0)
@@ -311,8 +320,8 @@ def preset(name, **overrides):
                                   precision=KT_PRECISION_MIXED)
     elif name == "grad512":
         c, cam = _observer_preset(512, 512, overrides.pop("spin", 0.7), overrides.pop("inclination", 70.0),
-                                  overrides.pop("r_obs", 60.0), 30.0, 1e-7, 20.0, 4, 0, 8.0,
-                                  precision=KT_PRECISION_MIXED, mdot=overrides.pop("mdot", 1.0))
+                                  overrides.pop("r_obs", 60.0), GRAD_FOV_DEG, 1e-7, 20.0, 4, 0, 8.0,
+                                  precision=KT_PRECISION_FP64, mdot=overrides.pop("mdot", 1.0))
     else:
         raise KeyError(f"unknown preset {name!r}")
     for key, value in overrides.items():

@@ -635,7 +635,7 @@ struct RayOut {
 // cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
 template <class S>
 void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
-               StepLog* log = nullptr) {
+               StepLog* log = nullptr, const double* replay_h = nullptr, int replay_n = 0) {
     // normalize (kerr_math.cuh:21-25)
     S dd = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
     S inv_n = 1.0 / sqrt_(dd);
@@ -676,7 +676,10 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
     S h(p.h0);
     State<S> fsal_k1;
     bool have_fsal = false;
-    for (int i = 0; i < p.max_steps && transmittance > p.cutoff; ++i) {
+    // replay (frozen step sequence): step i uses replay_h[i] in one
+    // attempt and the ray stops after replay_n steps
+    const int step_limit = replay_h ? std::min(p.max_steps, replay_n) : p.max_steps;
+    for (int i = 0; i < step_limit && transmittance > p.cutoff; ++i) {
         State<S> k1, k2, k3, k4, k5, k6, k7, ns;
         // tolerance scaled by the potential (kerr_trace.cuh:67-73), D1 floor
         S r_curr = ks_r(s.x, s.y, s.z, p.aa);
@@ -692,7 +695,7 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
         bool k1_ready = false;
         if (have_fsal) { k1 = fsal_k1; k1_ready = true; }
         while (!accepted && attempts < p.max_attempts) {
-            S hs = h;
+            S hs = replay_h ? S(replay_h[i]) : h;
             if (!k1_ready) { k1 = get_derivs(s, pt, p); k1_ready = true; st.rhs++; }
             Attempt<S> A;
             dp_attempt(s, pt, hs, tol, k1, p, A);
@@ -700,7 +703,7 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
             S error = A.error;
             st.rhs += 6;
             st.attempts++;
-            if (error <= 1.0) accepted = true;
+            if (error <= 1.0 || replay_h) accepted = true;
             // controller without clamps (kerr_trace.cuh:119); the step
             // sequence is data: no tangent flows through h
             h = S(val(hs) * 0.9 * pow_m02(val(error)));
@@ -879,6 +882,28 @@ inline uint32_t to_byte(double v) {
 using namespace kto;
 
 static thread_local char g_err[256];
+
+template <class S>
+static void observer_basis(S incl, S r_obs, S pos[3], S fwd[3], S rgt[3], S up[3]) {
+    S si = sin_(incl), ci = cos_(incl);
+    pos[0] = r_obs * si; pos[1] = r_obs * ci; pos[2] = S(0.0);
+    fwd[0] = -si; fwd[1] = -ci; fwd[2] = S(0.0);
+    rgt[0] = S(0.0); rgt[1] = S(0.0); rgt[2] = S(-1.0);
+    up[0] = -ci; up[1] = si; up[2] = S(0.0);
+}
+
+template <class S>
+static int grad_params(const kt_config* cfg, const S theta[4], Params<S>& p) {
+    kt_config c = *cfg;
+    c.spin = val(theta[0]);
+    c.mdot_scale = val(theta[3]);
+    c.escape_radius = val(theta[2]);
+    if (c.spp != 1) { std::snprintf(g_err, sizeof g_err, "differentiable renders need spp = 1"); return KT_ERR_INVALID_CONFIG; }
+    const char* why = nullptr;
+    int rc = derive<S>(c, theta[0], theta[3], p, &why);
+    if (rc != KT_OK) std::snprintf(g_err, sizeof g_err, "%s", why ? why : "invalid config");
+    return rc;
+}
 
 extern "C" {
 
@@ -1091,6 +1116,121 @@ int kto_trace_dir(const kt_config* cfg, const double cam[3], const double dir[3]
         if (n_log) *n_log = (int)(L.rows.size() / 8);
     }
     return ro.cls;
+}
+
+// ------------------------------------------------ differentiable renders
+// The differentiable scene: an observer at distance r_obs and inclination
+// i (radians) from the spin axis looking at the hole (pos = r (sin i,
+// cos i, 0), fwd = -pos / r, rgt = (0, 0, -1), up = (-cos i, sin i, 0)),
+// vertical FOV fov_deg, escape radius = r_obs (the camera sits on the
+// escape sphere), spin a and accretion-rate scale mdot; theta = (a, i,
+// r_obs, mdot).  spp must be 1.
+// Pixel block [x0, x0+w) x [y0, y0+h) of the differentiable scene with
+// forward-mode tangents: hdr3[w*h*3] (exposure applied), tan[w*h*3*4]
+// (d hdr_c / d theta_k at [pixel][c][k]), cls[w*h], nsteps[w*h].
+int kto_render_grad(const kt_config* cfg, const double theta[4], double fov_deg, int x0, int y0, int w, int h,
+                    int threads, double* hdr3, double* tan, uint8_t* cls, uint32_t* nsteps) {
+    typedef Dual<4> D;
+    D th[4];
+    for (int k = 0; k < 4; ++k) th[k] = D::seed(theta[k], k);
+    Params<D> p;
+    int rc = grad_params(cfg, th, p);
+    if (rc != KT_OK) return rc;
+    if (x0 < 0 || y0 < 0 || w <= 0 || h <= 0 || x0 + w > p.W || y0 + h > p.H) {
+        std::snprintf(g_err, sizeof g_err, "bad pixel block"); return KT_ERR_INVALID_ARGUMENT;
+    }
+    D pos[3], fwd[3], rgt[3], up[3];
+    observer_basis(th[1], th[2], pos, fwd, rgt, up);
+    const double fov_scale = std::tan(fov_deg * kPi / 360.0);
+    int nthreads = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            int row = next.fetch_add(1);
+            if (row >= h) break;
+            for (int xi = 0; xi < w; ++xi) {
+                RayStats st;
+                D rgb[3];
+                int c = 0;
+                bool o = false;
+                double oT = 0.0;
+                trace_pixel(pos, fwd, rgt, up, D(fov_scale), x0 + xi, y0 + row, p, rgb, &c, st, &o, &oT);
+                size_t idx = (size_t)row * w + xi;
+                for (int ch = 0; ch < 3; ++ch) {
+                    hdr3[idx * 3 + ch] = rgb[ch].v;
+                    for (int k = 0; k < 4; ++k) tan[(idx * 3 + ch) * 4 + k] = rgb[ch].d[k];
+                }
+                if (cls) cls[idx] = (uint8_t)c;
+                if (nsteps) nsteps[idx] = st.steps;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return KT_OK;
+}
+
+// Frozen-step finite-difference support: each pixel of the block is traced
+// at theta0 (adaptive, recording its committed step sizes) and then at
+// theta1 replaying exactly those steps.  hdr0/hdr1[w*h*3]; same[w*h] = 1
+// when both runs end alike (same class byte and step count).
+int kto_render_frozen(const kt_config* cfg, const double theta0[4], const double theta1[4], double fov_deg, int x0,
+                      int y0, int w, int h, int threads, double* hdr0, double* hdr1, uint8_t* same) {
+    Params<double> p0, p1;
+    int rc = grad_params(cfg, theta0, p0);
+    if (rc == KT_OK) rc = grad_params(cfg, theta1, p1);
+    if (rc != KT_OK) return rc;
+    if (x0 < 0 || y0 < 0 || w <= 0 || h <= 0 || x0 + w > p0.W || y0 + h > p0.H) {
+        std::snprintf(g_err, sizeof g_err, "bad pixel block"); return KT_ERR_INVALID_ARGUMENT;
+    }
+    double pos0[3], fwd0[3], rgt0[3], up0[3], pos1[3], fwd1[3], rgt1[3], up1[3];
+    observer_basis(theta0[1], theta0[2], pos0, fwd0, rgt0, up0);
+    observer_basis(theta1[1], theta1[2], pos1, fwd1, rgt1, up1);
+    const double fov_scale = std::tan(fov_deg * kPi / 360.0);
+    int nthreads = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int> next(0);
+    auto dir_of = [&](const Params<double>& p, const double* fwd, const double* rgt, const double* up, int x, int y,
+                      double dir[3]) {
+        uint32_t pixel_index = (uint32_t)(y * p.W + x);
+        uint64_t base = (uint64_t)pixel_index << 32;
+        double rx = rand01(base), ry = rand01(base ^ 0xda3e39cb94b95bdbULL);
+        double u = (((double)x + rx) * p.inv_w_2 - 1.0) * p.aspect * fov_scale;
+        double v = (1.0 - ((double)y + ry) * p.inv_h_2) * fov_scale;
+        for (int c = 0; c < 3; ++c) dir[c] = u * rgt[c] + v * up[c] + fwd[c];
+    };
+    auto work = [&]() {
+        for (;;) {
+            int row = next.fetch_add(1);
+            if (row >= h) break;
+            for (int xi = 0; xi < w; ++xi) {
+                const int x = x0 + xi, y = y0 + row;
+                size_t idx = (size_t)row * w + xi;
+                double dir[3];
+                dir_of(p0, fwd0, rgt0, up0, x, y, dir);
+                RayOut<double> r0, r1;
+                RayStats s0, s1;
+                StepLog L;
+                L.max_steps = (size_t)p0.max_steps;
+                trace_ray(pos0, dir, p0, r0, s0, &L);
+                std::vector<double> hs(L.rows.size() / 8);
+                for (size_t i = 0; i < hs.size(); ++i) hs[i] = L.rows[i * 8];
+                dir_of(p1, fwd1, rgt1, up1, x, y, dir);
+                trace_ray(pos1, dir, p1, r1, s1, nullptr, hs.data(), (int)hs.size());
+                for (int c = 0; c < 3; ++c) {
+                    hdr0[idx * 3 + c] = r0.col[c] * p0.exposure;
+                    hdr1[idx * 3 + c] = r1.col[c] * p1.exposure;
+                }
+                same[idx] = (r0.cls == r1.cls && s0.steps == s1.steps) ? 1 : 0;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return KT_OK;
 }
 
 // Bloom + ACES + sRGB + packing (cuda/kernel.cu:48-84, 144-251) on a float4

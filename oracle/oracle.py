@@ -48,6 +48,12 @@ def lib():
         L.kto_ks_r.argtypes, L.kto_ks_r.restype = [d, d, d, d], d
         L.kto_dense.argtypes, L.kto_dense.restype = [pd, d, d], d
         L.kto_dense_dt.argtypes, L.kto_dense_dt.restype = [pd, d, d], d
+        L.kto_render_grad.argtypes = [C.POINTER(KtConfig), pd, d, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      pd, pd, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]
+        L.kto_render_grad.restype = C.c_int
+        L.kto_render_frozen.argtypes = [C.POINTER(KtConfig), pd, pd, d, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, pd, pd, C.POINTER(C.c_uint8)]
+        L.kto_render_frozen.restype = C.c_int
         L.kto_disk_root.argtypes, L.kto_disk_root.restype = [pd, d], d
         L.kto_fbm.argtypes, L.kto_fbm.restype = [d, d, i32, u64], d
         L.kto_gradient_noise.argtypes, L.kto_gradient_noise.restype = [d, d, u64], d
@@ -117,6 +123,37 @@ def disk_root(y0, y1, k1, k3, k4, k5, k6, k7, h):
     """True-root crossing parameter t in [0, 1] of the dense output's y."""
     c = np.array([y0, y1, k1, k3, k4, k5, k6, k7], dtype=np.float64)
     return lib().kto_disk_root(_dptr(c), h)
+
+
+def _block(cfg, block):
+    return (0, 0, cfg.width, cfg.height) if block is None else tuple(int(v) for v in block)
+
+
+def render_grad(cfg, theta, fov_deg, block=None, threads=0):
+    """Differentiable scene (observer at theta = (a, i [rad], r_obs, mdot),
+    escape radius r_obs) with forward-mode tangents: returns hdr [h, w, 3],
+    tangents [h, w, 3, 4] (d hdr_c / d theta_k), cls [h, w], nsteps [h, w]."""
+    x0, y0, w, h = _block(cfg, block)
+    th = np.asarray(theta, dtype=np.float64)
+    hdr = np.zeros((h, w, 3)); tan = np.zeros((h, w, 3, 4))
+    cls = np.zeros((h, w), dtype=np.uint8); ns = np.zeros((h, w), dtype=np.uint32)
+    rc = lib().kto_render_grad(C.byref(cfg), _dptr(th), fov_deg, x0, y0, w, h, threads, _dptr(hdr), _dptr(tan),
+                               cls.ctypes.data_as(C.POINTER(C.c_uint8)), ns.ctypes.data_as(C.POINTER(C.c_uint32)))
+    _check(rc)
+    return hdr, tan, cls, ns
+
+
+def render_frozen(cfg, theta0, theta1, fov_deg, block=None, threads=0):
+    """Each pixel traced adaptively at theta0, then at theta1 replaying the
+    same committed step sizes.  Returns hdr0, hdr1 [h, w, 3] and same [h, w]
+    (True where both runs end with the same class and step count)."""
+    x0, y0, w, h = _block(cfg, block)
+    t0 = np.asarray(theta0, dtype=np.float64); t1 = np.asarray(theta1, dtype=np.float64)
+    h0 = np.zeros((h, w, 3)); h1 = np.zeros((h, w, 3)); same = np.zeros((h, w), dtype=np.uint8)
+    rc = lib().kto_render_frozen(C.byref(cfg), _dptr(t0), _dptr(t1), fov_deg, x0, y0, w, h, threads, _dptr(h0),
+                                 _dptr(h1), same.ctypes.data_as(C.POINTER(C.c_uint8)))
+    _check(rc)
+    return h0, h1, same.astype(bool)
 
 
 def fbm(u, v, octaves, seed=0):
