@@ -268,6 +268,44 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr,
                          uint8_t* const* hit_class, void* stream);
 kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out);
 
+/* ---- differentiable renderer (new: the reference has no gradients) -----
+ * The observer scene as a function of theta = {spin a, inclination i [rad],
+ * observer distance r_obs [M], accretion-rate scale mdot}: camera at
+ * r_obs (sin i, cos i, 0) looking at the hole (rgt = (0, 0, -1)), vertical
+ * FOV opts->fov_deg, escape radius = r_obs, mass and every other knob from
+ * cfg (FP64 precision, exact right-hand side and spp = 1 are required).
+ * kt_grad_forward traces the frame with the FP64 tracer and logs every
+ * ray's committed steps (a state checkpoint every checkpoint_interval
+ * steps); kt_grad_backward returns, for the frame of the last forward,
+ *   grad[k] = sum over pixels and channels of dl_dhdr * d hdr / d theta_k
+ * with the step sequence and every event (crossing, horizon, escape,
+ * table cell) held fixed.  grad: device double[4]; grad_px: NULL or device
+ * double[W*H*4] per-pixel contributions; dl_dhdr: device float4[W*H]
+ * (alpha ignored).  Errors are read with kt_last_error(NULL, ...). */
+typedef struct kt_grad_opts {
+    double fov_deg;              /* vertical field of view */
+    int32_t checkpoint_interval; /* committed steps per checkpoint, 1..256 */
+    int32_t log_steps_per_ray;   /* step-log room: W*H*(1 + ceil(this / interval)) chunks */
+} kt_grad_opts;
+typedef struct kt_grad_stats {
+    double forward_ms;           /* HIP events around the recording trace */
+    double backward_ms;          /* HIP events around the adjoint pass and its reduction */
+    uint64_t rhs_evals;          /* of the forward trace */
+    uint64_t attempts;
+    uint32_t chunks;             /* step-log chunks in use (W*H first chunks + pool) */
+    uint32_t log_overflow;
+    uint32_t checkpoint_interval;
+    uint32_t reserved;
+} kt_grad_stats;
+typedef struct kt_grad kt_grad;
+kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad** out);
+void kt_grad_destroy(kt_grad* g);
+kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr,
+                          uint8_t* hit_class, void* stream);
+kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad,
+                           double* grad_px, void* stream);
+kt_status kt_grad_get_stats(kt_grad* g, kt_grad_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

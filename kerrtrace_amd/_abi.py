@@ -147,3 +147,24 @@ class KtFrameView(C.Structure):
         ("reserved", C.c_uint32),
         ("data", C.POINTER(C.c_uint32)),
     ]
+
+
+class KtGradOpts(C.Structure):
+    _fields_ = [
+        ("fov_deg", C.c_double),
+        ("checkpoint_interval", C.c_int32),
+        ("log_steps_per_ray", C.c_int32),
+    ]
+
+
+class KtGradStats(C.Structure):
+    _fields_ = [
+        ("forward_ms", C.c_double),
+        ("backward_ms", C.c_double),
+        ("rhs_evals", C.c_uint64),
+        ("attempts", C.c_uint64),
+        ("chunks", C.c_uint32),
+        ("log_overflow", C.c_uint32),
+        ("checkpoint_interval", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]

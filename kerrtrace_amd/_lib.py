@@ -6,8 +6,8 @@ fails loudly when it is missing: there is no CPU fallback.
 import ctypes as C
 import os
 
-from ._abi import (KtCamera, KtConfig, KtDerived, KtFrameView, KtLaunchOpts,
-                   KtTraceStats)
+from ._abi import (KtCamera, KtConfig, KtDerived, KtFrameView, KtGradOpts,
+                   KtGradStats, KtLaunchOpts, KtTraceStats)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkerrtrace.so")
 
@@ -40,6 +40,11 @@ SIGNATURES = {
     "kt_batch_set_launch_opts": ([_vp, C.POINTER(KtLaunchOpts)], _i32),
     "kt_batch_trace": ([_vp, C.POINTER(_vp), C.POINTER(_vp), _vp], _i32),
     "kt_batch_get_stats": ([_vp, C.POINTER(KtTraceStats)], _i32),
+    "kt_grad_create": ([C.POINTER(KtConfig), C.POINTER(KtGradOpts), C.POINTER(_vp)], _i32),
+    "kt_grad_destroy": ([_vp], None),
+    "kt_grad_forward": ([_vp, _pd, _vp, _vp, _vp], _i32),
+    "kt_grad_backward": ([_vp, _vp, _vp, _vp, _vp], _i32),
+    "kt_grad_get_stats": ([_vp, C.POINTER(KtGradStats)], _i32),
 }
 
 _lib = None

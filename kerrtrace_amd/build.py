@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkerrtrace.so")
-SOURCES = ["kt_trace.hip", "kt_mixed.hip", "kt_schedule.hip", "kt_post.hip", "kt_host.cpp"]
+SOURCES = ["kt_trace.hip", "kt_mixed.hip", "kt_schedule.hip", "kt_post.hip", "kt_grad.hip", "kt_host.cpp"]
 
 # -ffp-contract=off: a*b+c keeps two roundings in both the FP64 kernels and
 # the host LUT code; fused operations are the explicit fma() calls.

@@ -1,0 +1,217 @@
+// Adjoint (reverse-mode) pass of the differentiable renderer on gfx950.
+//
+// The forward pass is the FP64 persistent tracer with its step log on
+// (trace_fp64_persistent<false, true>): per ray, a checkpoint of the state
+// every K committed steps, every committed step size and the transmittance
+// before every step.  Here one lane takes one pixel and walks its chunks
+// from the last to the first: it re-integrates the chunk's steps from the
+// checkpoint with the logged step sizes (the same arithmetic as the
+// forward pass, so the states are bit-identical), keeps them in its slice
+// of a scratch buffer, then runs the chunk's steps backwards:
+//   * escape: the sky term adds sky_int * Tr (piecewise constant in the
+//     direction), so it only seeds the transmittance adjoint;
+//   * crossing: reverse compositing (col += c Tr, Tr *= 1 - alpha), the
+//     emission's dual-number pass over its 9 inputs, then the dense output
+//     and its root;
+//   * the DP5(4) update: six right-hand-side VJPs (kt_adjoint.hpp).
+// At the ray's start the adjoint of (s0, pt, kt) is contracted with their
+// dual-number tangents in spin, inclination and distance.  Each pixel's
+// four gradient contributions go to grad_px; a deterministic reduction
+// sums them.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_adjoint.hpp"
+#include "kt_internal.hpp"
+#include "kt_math.hpp"
+#include "kt_tracer_common.hpp"
+
+namespace kt {
+
+// Per-lane scratch: element j of lane `tid` at sc[j * stride]; the first
+// 6 K elements hold the segment's states, the next 30 the stage inputs of
+// the step being reversed.
+struct LaneScratch {
+    double* p;
+    uint32_t stride;
+    __device__ __forceinline__ void put6(uint32_t j, const V6& v) const {
+        p[(size_t)(j + 0) * stride] = v.x;
+        p[(size_t)(j + 1) * stride] = v.y;
+        p[(size_t)(j + 2) * stride] = v.z;
+        p[(size_t)(j + 3) * stride] = v.px;
+        p[(size_t)(j + 4) * stride] = v.py;
+        p[(size_t)(j + 5) * stride] = v.pz;
+    }
+    __device__ __forceinline__ V6 get6(uint32_t j) const {
+        return V6{p[(size_t)(j + 0) * stride], p[(size_t)(j + 1) * stride], p[(size_t)(j + 2) * stride],
+                  p[(size_t)(j + 3) * stride], p[(size_t)(j + 4) * stride], p[(size_t)(j + 5) * stride]};
+    }
+};
+
+// One pixel's gradient contribution: out = d(sum_c w_c hdr_c)/d theta.
+__device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, int py, const double w[3],
+                            const LaneScratch& sc, double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    const GradLog& LG = P.log;
+    const size_t pix = (size_t)py * (size_t)P.W + (size_t)px;
+    const uint32_t N = LG.nsteps[pix];
+    if (N == 0) return;  // no step logged (or the log overflowed)
+    const uint32_t K = LG.K;
+    const Geo g{P.a, P.aa, P.M, true};
+    const double pt = LG.pk[2 * pix], kt = LG.pk[2 * pix + 1];
+    const uint8_t cls = P.cls ? P.cls[pix] : 0;
+    // colour adjoint W = d loss / d colour
+    const double W0 = w[0] * G.w_scale, W1 = w[1] * G.w_scale, W2 = w[2] * G.w_scale;
+    // escape: col += sky_int * Tr_end on each channel where the grid is hit
+    double lamTr = ((cls & 3) == 2 && (cls & 8)) ? P.sky_int * (W0 + W1 + W2) : 0.0;
+    V6 lam = v6_zero();
+    double lam_pt = 0.0, lam_kt = 0.0, g_a = 0.0, g_m = 0.0;
+    uint32_t chunk = LG.head[pix];
+    uint32_t seg_start = ((N - 1) / K) * K;
+    const uint32_t jY = 6 * K;
+    for (;;) {
+        const double* ch = LG.arena + (size_t)chunk * LG.stride;
+        const uint32_t len = min(K, N - seg_start);
+        V6 s{ch[0], ch[1], ch[2], ch[3], ch[4], ch[5]};
+        sc.put6(0, s);
+        for (uint32_t u = 1; u < len; ++u) {
+            s = dp_step(s, ch[8 + u - 1], pt, g, nullptr).ns;
+            sc.put6(6 * u, s);
+        }
+        for (int u = (int)len - 1; u >= 0; --u) {
+            const V6 sn = sc.get6(6 * (uint32_t)u);
+            const double hs = ch[8 + u];
+            const double Tr_n = ch[8 + K + u];
+            V6 Y[5];
+            const StepOut so = dp_step(sn, hs, pt, g, Y);
+            for (int i = 0; i < 5; ++i) sc.put6(jY + 6 * i, Y[i]);
+            const V6 ns = so.ns;
+            const double r_now = ks_r(ns.x, ns.y, ns.z, P.aa);
+            const bool horizon = r_now < P.rh_eps;
+            const bool crossing = !horizon && ((sn.y < 0.0 && ns.y >= 0.0) || (sn.y > 0.0 && ns.y <= 0.0));
+            V6 lk[6];
+            V6 lam_s_dir = v6_zero();
+            bool have_lk = false;
+            if (crossing) {
+                const V6 k7 = rhs(ns, pt, g);
+                V6 dsum = so.dacc6;
+                dsum.x = dsum.x + Dense::d7 * k7.x;
+                dsum.y = dsum.y + Dense::d7 * k7.y;
+                dsum.z = dsum.z + Dense::d7 * k7.z;
+                dsum.px = dsum.px + Dense::d7 * k7.px;
+                dsum.py = dsum.py + Dense::d7 * k7.py;
+                dsum.pz = dsum.pz + Dense::d7 * k7.pz;
+                const Crossing hit = crossing_hit(sn, ns, so.k1, k7, dsum, hs, P.disk_root_mode);
+                const ShadeD sv = shade_value(P, G, hit, pt, kt);
+                if (sv.lit) {
+                    // col += c Tr_n;  Tr_{n+1} = Tr_n (1 - alpha)
+                    const double c0 = sv.c[0].v, c1 = sv.c[1].v, c2 = sv.c[2].v, al = sv.alpha.v;
+                    const double lam_c[3] = {W0 * Tr_n, W1 * Tr_n, W2 * Tr_n};
+                    const double lam_alpha = -lamTr * Tr_n;
+                    lamTr = W0 * c0 + W1 * c1 + W2 * c2 + lamTr * (1.0 - al);
+                    double lam_hit[5];
+                    shade_reverse(P, G, hit, pt, kt, lam_c, lam_alpha, lam_hit, lam_pt, lam_kt, g_a, g_m);
+                    for (int i = 0; i < 6; ++i) lk[i] = v6_zero();
+                    V6 lam_ns_dir = v6_zero();
+                    crossing_reverse(sn, ns, so.k1, k7, dsum, hs, P.disk_root_mode, hit.t, lam_hit, lam_s_dir,
+                                     lam_ns_dir, lk);
+                    v6_add(lam, lam_ns_dir);
+                    have_lk = true;
+                }
+            }
+            for (int i = 0; i < 5; ++i) Y[i] = sc.get6(jY + 6 * i);
+            lam = dp_step_reverse(sn, ns, Y, hs, pt, g, lam, have_lk ? lk : nullptr, lam_s_dir, lam_pt, g_a);
+        }
+        if (seg_start == 0) break;
+        chunk = (uint32_t)__double_as_longlong(ch[6]);
+        seg_start -= K;
+    }
+    // ray start: s0, pt, kt as functions of spin, inclination, distance
+    for (int dir = 0; dir < 3; ++dir) {
+        const StartD st = start_d(P, G, px, py, dir);
+        double d = lam.x * st.s[0].d + lam.y * st.s[1].d + lam.z * st.s[2].d + lam.px * st.s[3].d +
+                   lam.py * st.s[4].d + lam.pz * st.s[5].d + lam_pt * st.pt.d + lam_kt * st.kt.d;
+        out[dir] += d;
+    }
+    out[0] += g_a;
+    out[3] += g_m;
+}
+
+// Persistent adjoint grid: each wave claims 64 work items (the forward
+// pass's tile order) with one atomic; each lane reverses one ray.
+__global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restrict__ Ps, GradConsts G,
+                                                     const float* __restrict__ dl_dhdr, double* __restrict__ grad_px,
+                                                     double* __restrict__ scratch, uint32_t scratch_stride,
+                                                     unsigned long long* __restrict__ queue, uint64_t total_items) {
+    const RenderParams& P = Ps[0];
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const LaneScratch sc{scratch + tid, scratch_stride};
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(queue, (unsigned long long)kWave);
+        base = __shfl(base, 0);
+        if (base >= total_items) break;
+        const uint64_t item = base + (uint64_t)lane;
+        int rid = 0, px = 0, py = 0;
+        if (item < total_items && decode_item<false>(Ps, 1, item, rid, px, py)) {
+            const size_t pix = (size_t)py * (size_t)P.W + (size_t)px;
+            const float4 wv = *reinterpret_cast<const float4*>(dl_dhdr + 4 * pix);
+            const double w[3] = {(double)wv.x, (double)wv.y, (double)wv.z};
+            double out[4];
+            ray_adjoint(P, G, px, py, w, sc, out);
+            double* o = grad_px + 4 * pix;
+            o[0] = out[0];
+            o[1] = out[1];
+            o[2] = out[2];
+            o[3] = out[3];
+        }
+    }
+}
+
+// Deterministic sum of the per-pixel contributions: one block of 256
+// threads, each summing a fixed strided subset, then a fixed tree.
+__global__ __launch_bounds__(256) void grad_reduce(const double* __restrict__ grad_px, uint32_t n_pix,
+                                                   double* __restrict__ out) {
+    __shared__ double part[4][256];
+    const int t = threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = (uint32_t)t; i < n_pix; i += 256)
+        for (int k = 0; k < 4; ++k) acc[k] += grad_px[4 * (size_t)i + k];
+    for (int k = 0; k < 4; ++k) part[k][t] = acc[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int k = 0; k < 4; ++k) part[k][t] += part[k][t + off];
+        __syncthreads();
+    }
+    if (t < 4) out[t] = part[t][0];
+}
+
+__global__ void zero_u64(unsigned long long* p) {
+    if (threadIdx.x == 0) *p = 0ull;
+}
+
+__global__ void reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t first_pool_chunk) {
+    if (threadIdx.x == 0) {
+        *pool_head = first_pool_chunk;
+        *overflow = 0u;
+    }
+}
+
+hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t first_pool_chunk, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_log, dim3(1), dim3(64), 0, stream, pool_head, overflow, first_pool_chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
+                                double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
+                                uint64_t total_items, uint32_t n_pix, double* grad_out, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, queue);
+    hipLaunchKernelGGL(grad_backward, dim3(threads / 256), dim3(256), 0, stream, d_params, G, dl_dhdr, grad_px,
+                       scratch, threads, queue, total_items);
+    hipLaunchKernelGGL(grad_reduce, dim3(1), dim3(256), 0, stream, grad_px, n_pix, grad_out);
+    return hipGetLastError();
+}
+
+}  // namespace kt

@@ -24,6 +24,8 @@
 #include <vector>
 
 #include "../../include/kerrtrace.h"
+#include "kt_adjoint.hpp"
+#include "kt_host_physics.hpp"
 #include "kt_internal.hpp"
 
 namespace kt {
@@ -38,6 +40,12 @@ hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, in
                          const float* d_wts, float intensity, int bloom_on, hipStream_t stream);
 hipError_t launch_cost_order(const RenderParams* d_params, int n_renders, int n_tiles, uint8_t* d_keys,
                              uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream);
+hipError_t launch_trace_fp64_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
+                                    DevCounters* d_counters, int blocks, hipStream_t stream);
+hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t first_pool_chunk, hipStream_t stream);
+hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
+                                double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
+                                uint64_t total_items, uint32_t n_pix, double* grad_out, hipStream_t stream);
 size_t bloom_h_lds_bytes(int R);
 size_t post_v_lds_bytes(int R, int TH);
 }  // namespace kt
@@ -48,40 +56,8 @@ const double kPi = 3.14159265358979323846;
 thread_local std::string g_last_error;
 
 // ------------------------------------------------------------ host physics
-// Prograde/retrograde ISCO (utils.rs:11-20).
-double calc_isco(double a_norm, bool prograde, double mass) {
-    double aa = a_norm * a_norm;
-    double z1 = std::fma(std::cbrt(1.0 - aa), std::cbrt(1.0 + a_norm) + std::cbrt(1.0 - a_norm), 1.0);
-    double z2 = std::sqrt(std::fma(3.0, aa, z1 * z1));
-    double sign = prograde ? -1.0 : 1.0;
-    double term_inside = (3.0 - z1) * std::fma(2.0, z2, 3.0 + z1);
-    return mass * (3.0 + z2 + sign * std::sqrt(std::fmax(term_inside, 0.0)));
-}
-
-// Novikov-Thorne flux factor exactly as the reference writes it
-// (utils.rs:21-52): of the three cubic-root log terms only those whose
-// root denominator is positive are kept.
-double nt_factor(double r, double a_norm, double r_isco, double inv_m) {
-    if (r <= r_isco) return 0.0;
-    double r_norm = r * inv_m;
-    double x = std::sqrt(r_norm);
-    double x_ms = std::sqrt(r_isco * inv_m);
-    double base = std::acos(-a_norm) * (1.0 / 3.0);
-    const double step = 2.0 * kPi / 3.0;
-    const double roots[3] = {2.0 * std::cos(base), 2.0 * std::cos(base - step), 2.0 * std::cos(base + step)};
-    double sum_log = 0.0;
-    for (int i = 0; i < 3; ++i) {
-        double xi = roots[i];
-        double denom = xi * (xi - roots[(i + 1) % 3]) * (xi - roots[(i + 2) % 3]);
-        if (denom <= 0.0) continue;
-        double coef = 3.0 * (xi - a_norm) * (xi - a_norm) / denom;
-        double v = (x - xi) / (x_ms - xi);
-        if (v > 0.0) sum_log = sum_log + coef * std::log(v);
-    }
-    double q = std::fma(1.5 * a_norm, -std::log(x / x_ms), x - x_ms) - sum_log;
-    double geo = r_norm * std::fma(2.0, a_norm, std::fma(r_norm, x, -(3.0 * x)));
-    return std::fmax(q / geo, 0.0);
-}
+using kt_host::calc_isco;
+using kt_host::nt_factor;
 
 // Multi-lobe Gaussian fit of the CIE 1931 colour-matching functions
 // (Wyman, Sloan and Shirley 2013) as used by blackbody.rs:6-25.
@@ -494,6 +470,100 @@ void release_batch(kt_batch* b) {
 bool same_blackbody(const kt_config& a, const kt_config& b) {
     return a.lut_size == b.lut_size && a.lut_max_temp == b.lut_max_temp && a.wavelength_start == b.wavelength_start &&
            a.wavelength_end == b.wavelength_end && a.wavelength_step == b.wavelength_step;
+}
+
+}  // namespace
+
+// ------------------------------------------------ differentiable renderer
+// The observer scene as a function of theta = (spin, inclination [rad],
+// r_obs, mdot): one recording FP64 trace per forward, one adjoint pass per
+// backward (kt_grad.hip).
+struct kt_grad {
+    kt_config cfg;
+    kt_grad_opts opts;
+    Derived d;
+    double theta[4] = {0, 0, 0, 0};
+    double d_rin_da = 0, dTs_da = 0, dTs_dm = 0;
+    std::vector<double> dlut;
+    int device = 0, cus = 256;
+    uint32_t n_pix = 0, K = 32, capacity = 0, stride = 0;
+    uint32_t threads = 0;  // adjoint grid
+    double* d_disk_lut = nullptr;
+    double* d_bb_lut = nullptr;
+    double* d_dlut = nullptr;
+    kt::RenderParams* h_params = nullptr;
+    kt::RenderParams* d_params = nullptr;
+    kt::DevCounters* d_counters = nullptr;
+    uint32_t* d_flag = nullptr;
+    uint8_t* d_cls = nullptr;
+    uint8_t* d_keys = nullptr;
+    uint32_t* d_hist = nullptr;
+    uint32_t* d_perm = nullptr;
+    double* d_arena = nullptr;
+    uint32_t* d_head = nullptr;
+    uint32_t* d_nsteps = nullptr;
+    double* d_pk = nullptr;
+    uint32_t* d_pool = nullptr;  // [0] pool head, [1] overflow
+    uint32_t* h_pool = nullptr;  // pinned copy
+    double* d_grad_px = nullptr;
+    double* d_scratch = nullptr;
+    unsigned long long* d_queue = nullptr;
+    hipEvent_t ev[4] = {};
+    hipEvent_t ev_params = nullptr;
+    bool params_pending = false, have_forward = false, have_backward = false;
+    hipStream_t last_stream = nullptr;
+    uint64_t items = 0;
+};
+
+namespace {
+
+void release_grad(kt_grad* g) {
+    if (!g) return;
+    void* bufs[] = {g->d_disk_lut, g->d_bb_lut, g->d_dlut, g->d_params, g->d_counters, g->d_flag, g->d_cls,
+                    g->d_keys, g->d_hist, g->d_perm, g->d_arena, g->d_head, g->d_nsteps, g->d_pk, g->d_pool,
+                    g->d_grad_px, g->d_scratch, g->d_queue};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (g->h_params) (void)hipHostFree(g->h_params);
+    if (g->h_pool) (void)hipHostFree(g->h_pool);
+    for (hipEvent_t e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (g->ev_params) (void)hipEventDestroy(g->ev_params);
+    delete g;
+}
+
+// Values and spin / mdot derivatives of everything the scene derives from
+// theta on the host: disk_inner, the disk LUT, temperature_scale.
+int derive_grad(kt_grad* g, const double theta[4], std::string& why) {
+    kt_config c = g->cfg;
+    c.spin = theta[0];
+    c.mdot_scale = theta[3];
+    c.escape_radius = theta[2];
+    int rc = derive(c, g->d, why);
+    if (rc != KT_OK) return rc;
+    using kt_host::HD;
+    const double M = c.mass, inv_m = 1.0 / M;
+    const HD a_norm{c.spin * inv_m, inv_m};
+    const HD rin = kt_host::calc_isco(a_norm, true, M);
+    g->d_rin_da = rin.d;
+    const int n = (int)c.lut_size;
+    g->dlut.assign(n, 0.0);
+    const HD span = HD{c.disk_outer_radius, 0.0} - rin;
+    int imax = 0;
+    for (int i = 0; i < n; ++i) {
+        HD r = kt_host::m_fma(HD{(double)i / (double)(n - 1), 0.0}, span, rin);
+        HD f = kt_host::nt_factor(r, a_norm, rin, inv_m);
+        if (f.v > 0.0) {
+            HD v = kt_host::m_sqrt(kt_host::m_sqrt(f));
+            g->dlut[i] = v.d;
+        }
+        if (g->d.disk_lut[i] > g->d.disk_lut[imax]) imax = i;
+    }
+    const double Ts = g->d.T_scale;
+    g->dTs_da = (c.disk_peak_temperature > 0.0 && g->d.lut_max > 0.0) ? -Ts * g->dlut[imax] / g->d.lut_max : 0.0;
+    g->dTs_dm = c.mdot_scale > 0.0 ? Ts / (4.0 * c.mdot_scale) : 0.0;
+    for (int k = 0; k < 4; ++k) g->theta[k] = theta[k];
+    return KT_OK;
 }
 
 }  // namespace
@@ -1046,4 +1116,211 @@ kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out) {
     return KT_OK;
 }
 
+
+kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad** out) {
+    if (!cfg || !opts || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (cfg->spp != 1) return set_err(nullptr, KT_ERR_INVALID_CONFIG, "the differentiable renderer needs spp = 1");
+    if (cfg->precision != KT_PRECISION_FP64 || cfg->rhs_mode != KT_RHS_HAMILTONIAN)
+        return set_err(nullptr, KT_ERR_INVALID_CONFIG,
+                       "the differentiable renderer needs FP64 precision and the exact (Hamiltonian) right-hand side");
+    if (!(opts->fov_deg > 0.0 && opts->fov_deg < 180.0) || opts->checkpoint_interval < 1 ||
+        opts->checkpoint_interval > 256 || opts->log_steps_per_ray < 1)
+        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "fov in (0, 180), checkpoint interval 1..256, log size >= 1");
+    kt_grad* g = new (std::nothrow) kt_grad();
+    if (!g) return set_err(nullptr, KT_ERR_DEVICE, "out of host memory");
+    g->cfg = *cfg;
+    g->opts = *opts;
+    std::string why;
+    int rc = derive(g->cfg, g->d, why);
+    if (rc != KT_OK) {
+        release_grad(g);
+        return set_err(nullptr, rc, why);
+    }
+    const uint64_t n_pix = (uint64_t)cfg->width * cfg->height;
+    g->K = (uint32_t)opts->checkpoint_interval;
+    g->stride = 8 + 2 * g->K;
+    const uint64_t per_ray = 1 + ((uint64_t)opts->log_steps_per_ray + g->K - 1) / g->K;
+    const uint64_t cap = n_pix * per_ray;
+    if (n_pix > 0x3FFFFFFFull || cap >= 0xFFFFFFF0ull) {
+        release_grad(g);
+        return set_err(nullptr, KT_ERR_CAPACITY, "frame or step log too large");
+    }
+    g->n_pix = (uint32_t)n_pix;
+    g->capacity = (uint32_t)cap;
+#define KT_GRAD_HIP(call, what)                   \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess) {                   \
+            int rc_ = hip_err(nullptr, e_, what); \
+            release_grad(g);                      \
+            return rc_;                           \
+        }                                         \
+    } while (0)
+    KT_GRAD_HIP(hipGetDevice(&g->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    KT_GRAD_HIP(hipGetDeviceProperties(&prop, g->device), "hipGetDeviceProperties");
+    g->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    const size_t n = cfg->lut_size;
+    KT_GRAD_HIP(hipMalloc(&g->d_disk_lut, n * sizeof(double)), "hipMalloc disk LUT");
+    KT_GRAD_HIP(hipMalloc(&g->d_dlut, n * sizeof(double)), "hipMalloc disk LUT tangent");
+    KT_GRAD_HIP(hipMalloc(&g->d_bb_lut, 3 * n * sizeof(double)), "hipMalloc blackbody LUT");
+    KT_GRAD_HIP(hipMemcpy(g->d_bb_lut, g->d.bb_lut.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice),
+                "upload blackbody LUT");
+    KT_GRAD_HIP(hipHostMalloc(&g->h_params, sizeof(kt::RenderParams), hipHostMallocDefault), "pinned params");
+    KT_GRAD_HIP(hipHostMalloc(&g->h_pool, 2 * sizeof(uint32_t), hipHostMallocDefault), "pinned pool flag");
+    KT_GRAD_HIP(hipMalloc(&g->d_params, sizeof(kt::RenderParams)), "hipMalloc params");
+    KT_GRAD_HIP(hipMalloc(&g->d_counters, sizeof(kt::DevCounters)), "hipMalloc counters");
+    KT_GRAD_HIP(hipMalloc(&g->d_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
+    KT_GRAD_HIP(hipMalloc(&g->d_cls, n_pix), "hipMalloc classes");
+    const size_t tiles = (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8);
+    KT_GRAD_HIP(hipMalloc(&g->d_keys, tiles), "hipMalloc cost keys");
+    KT_GRAD_HIP(hipMalloc(&g->d_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_GRAD_HIP(hipMalloc(&g->d_perm, tiles * sizeof(uint32_t)), "hipMalloc tile order");
+    KT_GRAD_HIP(hipMalloc(&g->d_arena, (size_t)cap * g->stride * sizeof(double)), "hipMalloc step log");
+    KT_GRAD_HIP(hipMalloc(&g->d_head, n_pix * sizeof(uint32_t)), "hipMalloc log heads");
+    KT_GRAD_HIP(hipMalloc(&g->d_nsteps, n_pix * sizeof(uint32_t)), "hipMalloc log steps");
+    KT_GRAD_HIP(hipMalloc(&g->d_pk, 2 * n_pix * sizeof(double)), "hipMalloc pt/kt");
+    KT_GRAD_HIP(hipMalloc(&g->d_pool, 2 * sizeof(uint32_t)), "hipMalloc pool head");
+    KT_GRAD_HIP(hipMalloc(&g->d_grad_px, 4 * n_pix * sizeof(double)), "hipMalloc per-pixel gradient");
+    g->threads = (uint32_t)std::max(256, (g->cus * 8 + 3) / 4 * 256);
+    const size_t sc = (size_t)(6 * g->K + 30) * g->threads;
+    KT_GRAD_HIP(hipMalloc(&g->d_scratch, sc * sizeof(double)), "hipMalloc adjoint scratch");
+    KT_GRAD_HIP(hipMalloc(&g->d_queue, sizeof(unsigned long long)), "hipMalloc queue");
+    for (auto& e : g->ev) KT_GRAD_HIP(hipEventCreate(&e), "event");
+    KT_GRAD_HIP(hipEventCreateWithFlags(&g->ev_params, hipEventDisableTiming), "event");
+#undef KT_GRAD_HIP
+    *out = g;
+    return KT_OK;
+}
+
+void kt_grad_destroy(kt_grad* g) {
+    if (!g) return;
+    if (g->last_stream || g->have_forward) (void)hipStreamSynchronize(g->last_stream);
+    release_grad(g);
+}
+
+kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr, uint8_t* hit_class, void* stream) {
+    if (!g || !theta || !hdr) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(theta[k])) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "theta is not finite");
+    if (!(theta[2] > 0.0) || !(theta[3] > 0.0))
+        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "r_obs and mdot must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    // the previous frame's work reads the buffers rewritten below
+    if (g->have_forward) {
+        hipError_t e = hipStreamSynchronize(g->last_stream);
+        if (e != hipSuccess) return hip_err(nullptr, e, "previous frame wait");
+    }
+    std::string why;
+    int rc = derive_grad(g, theta, why);
+    if (rc != KT_OK) return set_err(nullptr, rc, why);
+    const size_t n = g->cfg.lut_size;
+    hipError_t e = hipMemcpy(g->d_disk_lut, g->d.disk_lut.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->d_dlut, g->dlut.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_err(nullptr, e, "LUT upload");
+    kt_config c = g->cfg;
+    c.spin = theta[0];
+    c.mdot_scale = theta[3];
+    c.escape_radius = theta[2];
+    kt_camera cam;
+    const double si = std::sin(theta[1]), ci = std::cos(theta[1]);
+    const double pos[3] = {theta[2] * si, theta[2] * ci, 0.0}, fwd[3] = {-si, -ci, 0.0};
+    const double rgt[3] = {0.0, 0.0, -1.0}, up[3] = {-ci, si, 0.0};
+    for (int k = 0; k < 3; ++k) {
+        cam.pos[k] = pos[k];
+        cam.fwd[k] = fwd[k];
+        cam.rgt[k] = rgt[k];
+        cam.up[k] = up[k];
+    }
+    cam.fov_scale = std::tan(g->opts.fov_deg * kPi / 360.0);
+    kt::RenderParams& P = *g->h_params;
+    const int tile = 8;
+    make_params(nullptr, c, g->d, cam, hdr, g->d_cls, g->d_flag, tile, P);
+    P.disk_lut = g->d_disk_lut;
+    P.bb_lut = g->d_bb_lut;
+    P.tile_perm = g->d_perm;
+    P.log.arena = g->d_arena;
+    P.log.head = g->d_head;
+    P.log.nsteps = g->d_nsteps;
+    P.log.pk = g->d_pk;
+    P.log.pool_head = g->d_pool;
+    P.log.overflow = g->d_pool + 1;
+    P.log.capacity = g->capacity;
+    P.log.K = g->K;
+    P.log.stride = g->stride;
+    g->items = (uint64_t)P.n_tiles * tile * tile;
+    e = hipMemcpyAsync(g->d_params, g->h_params, sizeof(kt::RenderParams), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "param upload");
+    e = kt::launch_cost_order(g->d_params, 1, (int)P.n_tiles, g->d_keys, g->d_hist, g->d_perm, s);
+    if (e == hipSuccess) e = kt::launch_reset(g->d_counters, g->d_flag, s);
+    if (e == hipSuccess) e = kt::launch_reset_log(g->d_pool, g->d_pool + 1, g->n_pix, s);
+    if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("forward set-up: ") + hipGetErrorString(e));
+    const int blocks = std::max(1, (g->cus * 8 + 3) / 4);
+    e = hipEventRecord(g->ev[0], s);
+    if (e == hipSuccess) e = kt::launch_trace_fp64_record(g->d_params, g->items, 8, g->d_counters, blocks, s);
+    if (e == hipSuccess) e = hipEventRecord(g->ev[1], s);
+    if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("recording trace: ") + hipGetErrorString(e));
+    if (hit_class) {
+        e = hipMemcpyAsync(hit_class, g->d_cls, g->n_pix, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_err(nullptr, e, "class copy");
+    }
+    g->last_stream = s;
+    g->have_forward = true;
+    g->have_backward = false;
+    return KT_OK;
+}
+
+kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad, double* grad_px, void* stream) {
+    if (!g || !dl_dhdr || !grad) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!g->have_forward) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "kt_grad_backward before kt_grad_forward");
+    hipStream_t s = (hipStream_t)stream;
+    // a step log that ran out of room cannot be reversed
+    hipError_t e = hipMemcpyAsync(g->h_pool, g->d_pool, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(nullptr, e, "step log check");
+    if (g->h_pool[1] != 0u)
+        return set_err(nullptr, KT_ERR_CAPACITY, "the step log overflowed: raise kt_grad_opts.log_steps_per_ray");
+    kt::GradConsts G;
+    G.dlut = g->d_dlut;
+    G.d_rin_da = g->d_rin_da;
+    G.dTs_da = g->dTs_da;
+    G.dTs_dm = g->dTs_dm;
+    G.incl = g->theta[1];
+    G.r_obs = g->theta[2];
+    G.fov_scale = std::tan(g->opts.fov_deg * kPi / 360.0);
+    G.w_scale = g->cfg.exposure_scale * (1.0 / (double)g->cfg.spp);
+    e = hipEventRecord(g->ev[2], s);
+    if (e == hipSuccess)
+        e = kt::launch_grad_backward(g->d_params, G, dl_dhdr, grad_px ? grad_px : g->d_grad_px, g->d_scratch,
+                                     g->threads, g->d_queue, g->items, g->n_pix, grad, s);
+    if (e == hipSuccess) e = hipEventRecord(g->ev[3], s);
+    if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("adjoint: ") + hipGetErrorString(e));
+    g->last_stream = s;
+    g->have_backward = true;
+    return KT_OK;
+}
+
+kt_status kt_grad_get_stats(kt_grad* g, kt_grad_stats* out) {
+    if (!g || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof *out);
+    if (!g->have_forward) return KT_OK;
+    hipError_t e = hipStreamSynchronize(g->last_stream);
+    if (e != hipSuccess) return hip_err(nullptr, e, "stream sync");
+    kt::DevCounters c;
+    uint32_t pool[2];
+    e = hipMemcpy(&c, g->d_counters, sizeof c, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(pool, g->d_pool, sizeof pool, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_err(nullptr, e, "read counters");
+    float ms = 0.0f;
+    out->forward_ms = hipEventElapsedTime(&ms, g->ev[0], g->ev[1]) == hipSuccess ? (double)ms : 0.0;
+    if (g->have_backward)
+        out->backward_ms = hipEventElapsedTime(&ms, g->ev[2], g->ev[3]) == hipSuccess ? (double)ms : 0.0;
+    out->rhs_evals = c.rhs_evals;
+    out->attempts = c.attempts;
+    out->chunks = pool[0];
+    out->log_overflow = pool[1];
+    out->checkpoint_interval = g->K;
+    return KT_OK;
+}
 }  // extern "C"

@@ -4,6 +4,28 @@
 
 namespace kt {
 
+// Step log of a recording trace (the differentiable renderer's forward
+// pass): per ray, chunks of K committed steps, each holding the state at
+// the chunk's first step (a checkpoint), the link to the ray's previous
+// chunk and, per step, the committed step size and the transmittance at
+// the step's start.  A ray's first chunk is chunk `pixel`; later chunks
+// come from a pool starting at W * H.
+struct GradLog {
+    double* arena;        // chunks of `stride` doubles
+    uint32_t* head;       // per pixel: the ray's last chunk
+    uint32_t* nsteps;     // per pixel: committed steps
+    double* pk;           // per pixel: pt, kt
+    uint32_t* pool_head;  // next free pool chunk
+    uint32_t* overflow;   // set when the pool is exhausted
+    uint32_t capacity;    // chunks in the arena
+    uint32_t K;           // steps per chunk (checkpoint interval)
+    uint32_t stride;      // doubles per chunk: 8 + 2 K
+    uint32_t reserved;
+};
+// chunk layout: [0..5] state, [6] previous chunk (bits; ~0 for none),
+// [7] unused, [8 + u] h of step u, [8 + K + u] transmittance before step u
+static constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
+
 // Everything one render's rays read, derived on the host from kt_config
 // (src/renderer/utils.rs:53-114, init.rs:43-73) plus the camera
 // (cuda/kernel.cu:88-97).  One record per render; a batched launch holds an
@@ -45,6 +67,7 @@ struct RenderParams {
     int32_t tile;             // tile edge in pixels (the same for every render of a launch)
     int32_t tiles_x;
     const uint32_t* tile_perm;  // visiting order of the launch's global tiles, or null
+    GradLog log;              // recording traces only
 };
 
 // Work counters, summed over the launch (one atomic per wave at exit).

@@ -14,10 +14,14 @@
 
 #include "kt_internal.hpp"
 
+// FP64 routines below are host+device so the adjoint's building blocks can
+// be exercised by a host build of the same source.
+#define KT_HDI __host__ __device__ __forceinline__
+
 namespace kt {
 
 // cuda/kerr_math.cuh:26-32: one 64-bit LCG step then the XSH-RR output.
-__device__ __forceinline__ uint32_t pcg32(uint64_t state) {
+KT_HDI uint32_t pcg32(uint64_t state) {
     uint64_t x = state * 6364136223846793005ULL + 1442695040888963407ULL;
     uint32_t xs = (uint32_t)(((x >> 18u) ^ x) >> 27u);
     uint32_t rot = (uint32_t)(x >> 59u);
@@ -28,7 +32,7 @@ __device__ __forceinline__ uint32_t pcg32(uint64_t state) {
 // x = m 2^e, m in [1,2), e = 5q + j: x^(-1/5) = m^(-1/5) 2^(-j/5) 2^(-q).
 // m^(-1/5): quadratic start (|rel. error| < 2e-3), then four Newton steps
 // y <- y (6 - m y^5) / 5 on y^-5 = m (error 2e-3 -> 1e-5 -> 4e-10 -> 5e-19).
-__device__ __forceinline__ double pow_m02(double x) {
+KT_HDI double pow_m02(double x) {
     if (x != x) return x;
     if (x == 0.0) return __builtin_huge_val();
     if (__builtin_isinf(x)) return x > 0.0 ? 0.0 : __builtin_nan("");
@@ -56,7 +60,7 @@ __device__ __forceinline__ double pow_m02(double x) {
 
 // cuda/kerr_math.cuh:50-59: Kerr-Schild radius from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0
 // (s_sq * rsqrt(s_sq) evaluated as one square root).
-__device__ __forceinline__ double ks_r(double x, double y, double z, double a2) {
+KT_HDI double ks_r(double x, double y, double z, double a2) {
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - a2;
     double s_sq = fma(u, u, 4.0 * (a2 * y) * y);
@@ -79,7 +83,7 @@ struct Geo {
 // dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p).  Products that
 // feed a sum are fused with explicit fma; the reciprocals 1/s, 1/r,
 // 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division.
-__device__ __forceinline__ V6 rhs(const V6& st, double pt, const Geo& g) {
+KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
     const double a2 = g.aa, a = g.a;
     double rho2 = fma(x, x, fma(y, y, z * z));
@@ -253,7 +257,7 @@ struct Dense {
     static constexpr double d6 = -1453857185.0 / 822651844.0;
     static constexpr double d7 = 69997945.0 / 29380423.0;
 };
-__device__ __forceinline__ double dense(double y0, double y1, double k1, double k3, double k4, double k5,
+KT_HDI double dense(double y0, double y1, double k1, double k3, double k4, double k5,
                                         double k6, double k7, double h, double t) {
     double r1 = y0;
     double r2 = y1 - y0;
@@ -266,7 +270,7 @@ __device__ __forceinline__ double dense(double y0, double y1, double k1, double 
     double t4 = t2 * t2;
     return r1 + t * (r2 + r3) - t2 * (r3 - r4 - r5) - t3 * (r4 + 2.0 * r5) + t4 * r5;
 }
-__device__ __forceinline__ double dense_dt(double y0, double y1, double k1, double k3, double k4, double k5,
+KT_HDI double dense_dt(double y0, double y1, double k1, double k3, double k4, double k5,
                                            double k6, double k7, double h, double t) {
     double r2 = y1 - y0;
     double r3 = y0 + h * k1 - y1;
@@ -282,7 +286,7 @@ __device__ __forceinline__ double dense_dt(double y0, double y1, double k1, doub
 // and dsum = d1 k1 + d3 k3 + d4 k4 + d5 k5 + d6 k6 + d7 k7 (summed left to
 // right, as dense() does), so a tracer can fold the sum while it runs the
 // stages instead of keeping every stage derivative.
-__device__ __forceinline__ double dense_s(double y0, double y1, double k1, double k7, double dsum, double h,
+KT_HDI double dense_s(double y0, double y1, double k1, double k7, double dsum, double h,
                                           double t) {
     double r1 = y0;
     double r2 = y1 - y0;
@@ -294,7 +298,7 @@ __device__ __forceinline__ double dense_s(double y0, double y1, double k1, doubl
     double t4 = t2 * t2;
     return r1 + t * (r2 + r3) - t2 * (r3 - r4 - r5) - t3 * (r4 + 2.0 * r5) + t4 * r5;
 }
-__device__ __forceinline__ double dense_s_dt(double y0, double y1, double k1, double k7, double dsum, double h,
+KT_HDI double dense_s_dt(double y0, double y1, double k1, double k7, double dsum, double h,
                                              double t) {
     double r2 = y1 - y0;
     double r3 = y0 + h * k1 - y1;
@@ -305,7 +309,7 @@ __device__ __forceinline__ double dense_s_dt(double y0, double y1, double k1, do
     return (r2 + r3) - 2.0 * t * (r3 - r4 - r5) - 3.0 * t2 * (r4 + 2.0 * r5) + 4.0 * t3 * r5;
 }
 // disk_root on dense_s (identical iteration).
-__device__ __forceinline__ double disk_root_s(double y0, double y1, double k1, double k7, double dsum, double h) {
+KT_HDI double disk_root_s(double y0, double y1, double k1, double k7, double dsum, double h) {
     double lo = 0.0, hi = 1.0, ylo = y0;
     double den = y0 - y1;
     double t = den != 0.0 ? y0 / den : 0.5;
@@ -331,7 +335,7 @@ __device__ __forceinline__ double disk_root_s(double y0, double y1, double k1, d
 
 // Root of the interpolant y(t) on [0,1]: secant start, Newton steps kept
 // inside the shrinking bracket, bisection when Newton leaves it.
-__device__ __forceinline__ double disk_root(double y0, double y1, double k1, double k3, double k4, double k5,
+KT_HDI double disk_root(double y0, double y1, double k1, double k3, double k4, double k5,
                                             double k6, double k7, double h) {
     double lo = 0.0, hi = 1.0, ylo = y0;
     double den = y0 - y1;
@@ -357,20 +361,20 @@ __device__ __forceinline__ double disk_root(double y0, double y1, double k1, dou
 }
 
 // cuda/kerr_math.cuh:172-179
-__device__ __forceinline__ double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
-__device__ __forceinline__ double lerp(double a, double b, double t) { return fma(t, b - a, a); }
+KT_HDI double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+KT_HDI double lerp(double a, double b, double t) { return fma(t, b - a, a); }
 
 // cuda/kerr_math.cuh:220-260: gradient noise on the integer lattice; the
 // gradient angle of a lattice point comes from pcg32 of its packed
 // coordinates (xor-ed with the seed mix, 0 = reference hash).
-__device__ __forceinline__ void lattice_gradient(int ix, int iy, uint64_t seed_mix, double& gx, double& gy) {
+KT_HDI void lattice_gradient(int ix, int iy, uint64_t seed_mix, double& gx, double& gy) {
     uint64_t st = ((uint64_t)(uint32_t)ix << 32) | (uint64_t)(uint32_t)iy;
     st ^= seed_mix;
     uint32_t hsh = pcg32(st);
     double angle = (double)hsh * (2.0 * 3.14159265358979323846 * 0x1p-32);
     sincos(angle, &gy, &gx);
 }
-__device__ __forceinline__ double gradient_noise(double u, double v, uint64_t seed_mix) {
+KT_HDI double gradient_noise(double u, double v, uint64_t seed_mix) {
     double fu = floor(u), fv = floor(v);
     int ix0 = (int)fu, iy0 = (int)fv;
     double fx = u - (double)ix0;
@@ -391,7 +395,7 @@ __device__ __forceinline__ double gradient_noise(double u, double v, uint64_t se
     return lerp(x1, x2, v_f);
 }
 // cuda/kerr_math.cuh:261-281
-__device__ __forceinline__ double fbm(double u, double v, int octaves, uint64_t seed_mix) {
+KT_HDI double fbm(double u, double v, int octaves, uint64_t seed_mix) {
     if (octaves <= 0) return 0.5;
     double sum = 0.0, amp = 1.0, freq = 1.0, amp_sum = 0.0;
     for (int i = 0; i < octaves; ++i) {

@@ -28,7 +28,9 @@
 
 namespace kt {
 
-template <bool BATCH>
+// RECORD: the differentiable renderer's forward pass, which also writes
+// the step log (RenderParams::log) the adjoint pass replays.
+template <bool BATCH, bool RECORD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_fp64_persistent(const RenderParams* __restrict__ Ps, int n_renders,
                                                              uint64_t total_items, int refill_threshold,
                                                              DevCounters* __restrict__ counters) {
@@ -42,6 +44,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     L.overflow = false;
     L.overflow_T = 0.0;
     L.ray_att = 0;
+    L.chunk = kNoChunk;
+    L.nrec = 0;
     LaneCounters cnt;
     bool drained = false;
     bool high_prio = false;
@@ -59,10 +63,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         if (L.finished) {
             L.finished = false;
             cnt.max_att = max(cnt.max_att, L.ray_att);
+            if (RECORD) record_end(L, prm<BATCH>(Ps, L.rid));
             end_sample(L, prm<BATCH>(Ps, L.rid), H);
         }
         // ---- refill idle lanes from the work queue (wave-uniform branch)
         refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
+
         if (drained && __ballot(L.active) == 0) break;
         // a wave holding a long-running ray (photon-ring orbits) issues
         // ahead of the other wave(s) on its SIMD, so its serial chain of
@@ -159,6 +165,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         if (accepted) cnt.accepted++;
         if (accepted) L.k1 = k7;  // FSAL
         L.have_k1 = accepted;
+        if (RECORD) record_step(L, prm<BATCH>(Ps, L.rid), s, hs, lane);
         commit(L, s, ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
     }
 
@@ -193,11 +200,19 @@ hipError_t launch_reset(DevCounters* c, uint32_t* error_flag, hipStream_t stream
 hipError_t launch_trace_fp64(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,
                              DevCounters* d_counters, int blocks, hipStream_t stream) {
     if (n_renders == 1)
-        hipLaunchKernelGGL(trace_fp64_persistent<false>, dim3(blocks), dim3(256), 0, stream, d_params, 1,
+        hipLaunchKernelGGL((trace_fp64_persistent<false, false>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
                            total_items, refill_threshold, d_counters);
     else
-        hipLaunchKernelGGL(trace_fp64_persistent<true>, dim3(blocks), dim3(256), 0, stream, d_params, n_renders,
-                           total_items, refill_threshold, d_counters);
+        hipLaunchKernelGGL((trace_fp64_persistent<true, false>), dim3(blocks), dim3(256), 0, stream, d_params,
+                           n_renders, total_items, refill_threshold, d_counters);
+    return hipGetLastError();
+}
+
+// The differentiable renderer's forward pass (one render, step log on).
+hipError_t launch_trace_fp64_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
+                                    DevCounters* d_counters, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((trace_fp64_persistent<false, true>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
+                       total_items, refill_threshold, d_counters);
     return hipGetLastError();
 }
 

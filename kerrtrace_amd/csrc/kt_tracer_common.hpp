@@ -281,6 +281,7 @@ struct Lane {
     double hx, hz, hpx, hpy, hpz;  // dense-output state at the equatorial crossing
     double r_now;     // Kerr-Schild radius of the committed state
     uint32_t ray_att;
+    uint32_t chunk, nrec;  // recording traces: current log chunk, committed steps
     bool active, finished, have_k1, overflow;
     bool post;        // the committed step still needs its crossing/escape/budget pass
     bool crossing;    // ... and it crossed the equator
@@ -382,6 +383,63 @@ __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const
         if (atomicCAS(P.error_flag, 0u, 1u) == 0u) P.error_flag[1] = __float_as_uint((float)L.overflow_T);
     }
     L.active = false;
+}
+
+// Recording traces: log committed step L.nrec (start state s, step size
+// hs, transmittance before the step); a new chunk (with s as its
+// checkpoint) starts every K steps.  Pool chunks are claimed with one
+// atomic per wave.
+__device__ __forceinline__ void record_step(Lane& L, const RenderParams& P, const V6& s, double hs, int lane) {
+    const GradLog& G = P.log;
+    const uint32_t u = L.nrec % G.K;
+    const bool fresh = u == 0;
+    const bool need_pool = fresh && L.nrec > 0 && L.chunk != kNoChunk;
+    const uint64_t want = __ballot(need_pool);
+    uint32_t base = 0;
+    if (want) {
+        const int leader = __ffsll((unsigned long long)want) - 1;
+        if (lane == leader) base = atomicAdd(G.pool_head, (uint32_t)__popcll(want));
+        base = __shfl(base, leader);
+    }
+    if (fresh) {
+        uint32_t c = kNoChunk;
+        if (L.nrec == 0) {
+            c = (uint32_t)(L.pix_y * P.W + L.pix_x);
+            double* pk = G.pk + 2 * (size_t)c;
+            pk[0] = L.pt;
+            pk[1] = L.kt;
+        } else if (need_pool) {
+            c = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+            if (c >= G.capacity) {
+                atomicOr(G.overflow, 1u);
+                c = kNoChunk;
+            }
+        }
+        if (c != kNoChunk) {
+            double* ch = G.arena + (size_t)c * G.stride;
+            ch[0] = s.x;
+            ch[1] = s.y;
+            ch[2] = s.z;
+            ch[3] = s.px;
+            ch[4] = s.py;
+            ch[5] = s.pz;
+            ch[6] = __longlong_as_double((long long)(L.nrec == 0 ? 0xFFFFFFFFull : (unsigned long long)L.chunk));
+        }
+        L.chunk = c;
+    }
+    if (L.chunk != kNoChunk) {
+        double* ch = G.arena + (size_t)L.chunk * G.stride;
+        ch[8 + u] = hs;
+        ch[8 + G.K + u] = L.Tr;
+    }
+    L.nrec += 1;
+}
+
+// Recording traces: a ray ended; its log head and step count.
+__device__ __forceinline__ void record_end(const Lane& L, const RenderParams& P) {
+    const size_t idx = (size_t)L.pix_y * (size_t)P.W + (size_t)L.pix_x;
+    P.log.head[idx] = L.chunk;
+    P.log.nsteps[idx] = L.chunk == kNoChunk ? 0u : L.nrec;
 }
 
 // Second half of a commit (cuda/kerr_trace.cuh:138-237 after the horizon
@@ -514,6 +572,8 @@ __device__ __forceinline__ void refill(Lane& L, Hot& H, const RenderParams* __re
         L.acc[0] = L.acc[1] = L.acc[2] = 0.0;
         L.overflow = false;
         L.ray_att = 0;
+        L.chunk = kNoChunk;
+        L.nrec = 0;
         L.active = true;
         L.post = false;
         start_sample(L, P, H);

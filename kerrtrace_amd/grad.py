@@ -1,0 +1,162 @@
+"""Differentiable renderer over the C ABI (kt_grad_*): the observer scene as
+a function of theta = (spin, inclination [rad], r_obs [M], mdot), with the
+adjoint pass exposed as a torch.autograd.Function.
+
+The gradient is that of the rendered linear HDR frame with the committed
+step sequence and every event (disk crossing, horizon, escape, lookup-table
+cell) held fixed: the derivative of the integrator the forward pass ran.
+The reference renders no gradients; this is an addition of this library.
+"""
+import ctypes as C
+
+from ._abi import KtGradOpts, KtGradStats
+from ._lib import check, lib
+from .config import GRAD_FOV_DEG
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("kerrtrace_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class GradRenderer:
+    """One differentiable renderer handle: forward(theta) then backward(dL/dhdr)."""
+
+    def __init__(self, cfg, fov_deg=GRAD_FOV_DEG, checkpoint_interval=32, log_steps_per_ray=512):
+        _torch()
+        self.cfg = cfg.copy()
+        self.opts = KtGradOpts(float(fov_deg), int(checkpoint_interval), int(log_steps_per_ray))
+        h = C.c_void_p()
+        check(lib().kt_grad_create(C.byref(self.cfg), C.byref(self.opts), C.byref(h)))
+        self._h = h
+        self.frame = 0  # forward passes so far; a backward belongs to the last one
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kt_grad_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.cfg.height, self.cfg.width)
+
+    def forward(self, theta, hdr=None, cls=None, stream=None):
+        """Trace the frame at theta (4 floats); returns (hdr float32 [H, W, 4],
+        cls uint8 [H, W] or None) on the current torch stream."""
+        torch = _torch()
+        th = (C.c_double * 4)(*[float(v) for v in theta])
+        H, W = self.shape
+        if hdr is None:
+            hdr = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().kt_grad_forward(self._h, th, C.c_void_p(hdr.data_ptr()),
+                                    C.c_void_p(cls.data_ptr() if cls is not None else 0),
+                                    C.c_void_p(s.cuda_stream)))
+        self.frame += 1
+        return hdr, cls
+
+    def backward(self, dl_dhdr, per_pixel=False, stream=None):
+        """d(sum dl_dhdr * hdr)/d theta for the last forward frame: float64 [4]
+        on the device (and the per-pixel contributions [H, W, 4] if asked)."""
+        torch = _torch()
+        H, W = self.shape
+        dl = dl_dhdr.detach()
+        if tuple(dl.shape) != (H, W, 4) or dl.dtype != torch.float32 or not dl.is_cuda:
+            dl = dl.to(device="cuda", dtype=torch.float32).reshape(H, W, 4)
+        dl = dl.contiguous()
+        grad = torch.empty(4, dtype=torch.float64, device="cuda")
+        gpx = torch.empty((H, W, 4), dtype=torch.float64, device="cuda") if per_pixel else None
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().kt_grad_backward(self._h, C.c_void_p(dl.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                     C.c_void_p(gpx.data_ptr() if gpx is not None else 0),
+                                     C.c_void_p(s.cuda_stream)))
+        return (grad, gpx) if per_pixel else grad
+
+    def stats(self):
+        st = KtGradStats()
+        check(lib().kt_grad_get_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+
+def _function():
+    torch = _torch()
+
+    class KerrRender(torch.autograd.Function):
+        """hdr = render(theta): float32 [H, W, 4] from float64 theta [4]."""
+
+        @staticmethod
+        def forward(ctx, theta, renderer):
+            values = theta.detach().to("cpu", torch.float64).tolist()
+            hdr, _ = renderer.forward(values)
+            ctx.renderer = renderer
+            ctx.frame = renderer.frame
+            ctx.theta_meta = (theta.device, theta.dtype)
+            return hdr
+
+        @staticmethod
+        def backward(ctx, grad_hdr):
+            r = ctx.renderer
+            if r.frame != ctx.frame:
+                raise RuntimeError("the renderer traced another frame after this one: run backward before the "
+                                   "next forward (the adjoint replays the last frame's step log)")
+            g = r.backward(grad_hdr)
+            device, dtype = ctx.theta_meta
+            return g.to(device=device, dtype=dtype), None
+
+    return KerrRender
+
+
+_KERR_RENDER = None
+
+
+def render(theta, renderer):
+    """Differentiable render: theta (tensor [4], requires_grad as wanted) ->
+    hdr float32 [H, W, 4]."""
+    global _KERR_RENDER
+    if _KERR_RENDER is None:
+        _KERR_RENDER = _function()
+    return _KERR_RENDER.apply(theta, renderer)
+
+
+def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), log_every=0):
+    """Fit theta to a target frame with Adam on the L2 loss over the linear
+    HDR colour channels.  Adam's step is about its learning rate in each
+    coordinate whatever the gradient's scale, so theta is optimised as
+    theta = step * z with learning rate 1 on z: `step` is the per-iteration
+    step size in spin, inclination [rad], r_obs [M] and mdot.  Returns
+    (theta [4] list, history of (loss, theta) before each update)."""
+    torch = _torch()
+    scale = torch.tensor(step, dtype=torch.float64)
+    z = (torch.tensor([float(v) for v in theta0], dtype=torch.float64) / scale).requires_grad_(True)
+    opt = torch.optim.Adam([z], lr=1.0)
+    target = target_hdr[..., :3].detach().to(torch.float64)
+    history = []
+    for it in range(iters):
+        opt.zero_grad()
+        theta = z * scale
+        hdr = render(theta, renderer)
+        loss = ((hdr[..., :3].to(torch.float64) - target) ** 2).sum()
+        loss.backward()
+        history.append((float(loss.detach()), [float(v) for v in theta.detach()]))
+        opt.step()
+        with torch.no_grad():
+            z[0].clamp_(0.0, 0.999 / step[0])
+            z[3].clamp_(min=1e-3 / step[3])
+        if log_every and it % log_every == 0:
+            print(f"[adam] it {it} loss {history[-1][0]:.6e} theta {history[-1][1]}", flush=True)
+    theta = (z * scale).detach()
+    return [float(v) for v in theta], history

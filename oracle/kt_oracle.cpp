@@ -1125,6 +1125,50 @@ int kto_trace_dir(const kt_config* cfg, const double cam[3], const double dir[3]
 // vertical FOV fov_deg, escape radius = r_obs (the camera sits on the
 // escape sphere), spin a and accretion-rate scale mdot; theta = (a, i,
 // r_obs, mdot).  spp must be 1.
+// Forward-mode directional derivatives for checks of adjoint code: the
+// right-hand side (kto_rhs_jvp) and one DP5(4) step's end point
+// (kto_step_jvp) at (s, pt; spin) along (ds, dpt; dspin), M from cfg,
+// exact right-hand side.  out6 = value, tout6 = directional derivative.
+static void jvp_params(const kt_config* cfg, double da, Params<Dual<1>>& p) {
+    typedef Dual<1> D;
+    p.a = D(cfg->spin);
+    p.a.d[0] = da;
+    p.aa = p.a * p.a;
+    p.M = D(cfg->mass);
+    p.rhs_exact = true;
+}
+int kto_rhs_jvp(const kt_config* cfg, const double s6[6], double pt, const double ds6[6], double dpt, double da,
+                double out6[6], double tout6[6]) {
+    typedef Dual<1> D;
+    Params<D> p;
+    jvp_params(cfg, da, p);
+    D v[6];
+    for (int k = 0; k < 6; ++k) { v[k] = D(s6[k]); v[k].d[0] = ds6[k]; }
+    D ptd(pt);
+    ptd.d[0] = dpt;
+    State<D> f = get_derivs(State<D>{v[0], v[1], v[2], v[3], v[4], v[5]}, ptd, p);
+    const D* o[6] = {&f.x, &f.y, &f.z, &f.px, &f.py, &f.pz};
+    for (int k = 0; k < 6; ++k) { out6[k] = o[k]->v; tout6[k] = o[k]->d[0]; }
+    return KT_OK;
+}
+int kto_step_jvp(const kt_config* cfg, const double s6[6], double pt, double h, const double ds6[6], double dpt,
+                 double da, double out6[6], double tout6[6]) {
+    typedef Dual<1> D;
+    Params<D> p;
+    jvp_params(cfg, da, p);
+    D v[6];
+    for (int k = 0; k < 6; ++k) { v[k] = D(s6[k]); v[k].d[0] = ds6[k]; }
+    D ptd(pt);
+    ptd.d[0] = dpt;
+    State<D> s{v[0], v[1], v[2], v[3], v[4], v[5]};
+    State<D> k1 = get_derivs(s, ptd, p);
+    Attempt<D> A;
+    dp_attempt(s, ptd, D(h), 1.0, k1, p, A);
+    const D* o[6] = {&A.ns.x, &A.ns.y, &A.ns.z, &A.ns.px, &A.ns.py, &A.ns.pz};
+    for (int k = 0; k < 6; ++k) { out6[k] = o[k]->v; tout6[k] = o[k]->d[0]; }
+    return KT_OK;
+}
+
 // Pixel block [x0, x0+w) x [y0, y0+h) of the differentiable scene with
 // forward-mode tangents: hdr3[w*h*3] (exposure applied), tan[w*h*3*4]
 // (d hdr_c / d theta_k at [pixel][c][k]), cls[w*h], nsteps[w*h].

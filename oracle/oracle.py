@@ -48,6 +48,10 @@ def lib():
         L.kto_ks_r.argtypes, L.kto_ks_r.restype = [d, d, d, d], d
         L.kto_dense.argtypes, L.kto_dense.restype = [pd, d, d], d
         L.kto_dense_dt.argtypes, L.kto_dense_dt.restype = [pd, d, d], d
+        L.kto_rhs_jvp.argtypes = [C.POINTER(KtConfig), pd, d, pd, d, d, pd, pd]
+        L.kto_rhs_jvp.restype = C.c_int
+        L.kto_step_jvp.argtypes = [C.POINTER(KtConfig), pd, d, d, pd, d, d, pd, pd]
+        L.kto_step_jvp.restype = C.c_int
         L.kto_render_grad.argtypes = [C.POINTER(KtConfig), pd, d, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       pd, pd, C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]
         L.kto_render_grad.restype = C.c_int
@@ -123,6 +127,23 @@ def disk_root(y0, y1, k1, k3, k4, k5, k6, k7, h):
     """True-root crossing parameter t in [0, 1] of the dense output's y."""
     c = np.array([y0, y1, k1, k3, k4, k5, k6, k7], dtype=np.float64)
     return lib().kto_disk_root(_dptr(c), h)
+
+
+def rhs_jvp(cfg, state6, pt, dstate6, dpt, dspin):
+    """Right-hand side and its derivative along (dstate, dpt, dspin)."""
+    s = np.asarray(state6, dtype=np.float64); ds = np.asarray(dstate6, dtype=np.float64)
+    out = np.zeros(6); tout = np.zeros(6)
+    _check(lib().kto_rhs_jvp(C.byref(cfg), _dptr(s), pt, _dptr(ds), dpt, dspin, _dptr(out), _dptr(tout)))
+    return out, tout
+
+
+def step_jvp(cfg, state6, pt, h, dstate6, dpt, dspin):
+    """End point of one DP5(4) step of size h and its derivative along
+    (dstate, dpt, dspin) with h fixed."""
+    s = np.asarray(state6, dtype=np.float64); ds = np.asarray(dstate6, dtype=np.float64)
+    out = np.zeros(6); tout = np.zeros(6)
+    _check(lib().kto_step_jvp(C.byref(cfg), _dptr(s), pt, h, _dptr(ds), dpt, dspin, _dptr(out), _dptr(tout)))
+    return out, tout
 
 
 def _block(cfg, block):

@@ -47,6 +47,7 @@ int main(void) {
          sizeof(kt_trace_stats), sizeof(kt_launch_opts), sizeof(kt_frame_view));
   printf("%zu %zu %zu %zu\n", offsetof(kt_config, noise_seed), offsetof(kt_config, bloom_threshold),
          offsetof(kt_config, precision), offsetof(kt_config, rhs_mode));
+  printf("%zu %zu %zu\n", sizeof(kt_grad_opts), sizeof(kt_grad_stats), offsetof(kt_grad_stats, chunks));
   return 0;
 }
 '''
@@ -63,8 +64,9 @@ def test_ctypes_layout_matches_header():
     v = _c_struct_sizes()
     assert v[:6] == [C.sizeof(_abi.KtConfig), C.sizeof(_abi.KtCamera), C.sizeof(_abi.KtDerived),
                      C.sizeof(_abi.KtTraceStats), C.sizeof(_abi.KtLaunchOpts), C.sizeof(_abi.KtFrameView)]
-    assert v[6:] == [_abi.KtConfig.noise_seed.offset, _abi.KtConfig.bloom_threshold.offset,
-                     _abi.KtConfig.precision.offset, _abi.KtConfig.rhs_mode.offset]
+    assert v[6:10] == [_abi.KtConfig.noise_seed.offset, _abi.KtConfig.bloom_threshold.offset,
+                       _abi.KtConfig.precision.offset, _abi.KtConfig.rhs_mode.offset]
+    assert v[10:] == [C.sizeof(_abi.KtGradOpts), C.sizeof(_abi.KtGradStats), _abi.KtGradStats.chunks.offset]
 
 
 def test_c_defaults_equal_python_defaults():
