@@ -67,6 +67,7 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
     V6 lam = v6_zero();
     double lam_pt = 0.0, lam_kt = 0.0, g_a = 0.0, g_m = 0.0;
     uint32_t chunk = LG.head[pix];
+    if (chunk >= LG.capacity) return;
     uint32_t seg_start = ((N - 1) / K) * K;
     const uint32_t jY = 6 * K;
     for (;;) {
@@ -124,6 +125,10 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
         }
         if (seg_start == 0) break;
         chunk = (uint32_t)__double_as_longlong(ch[6]);
+        if (chunk >= LG.capacity) {  // a broken chain: no contribution
+            out[0] = out[1] = out[2] = out[3] = 0.0;
+            return;
+        }
         seg_start -= K;
     }
     // ray start: s0, pt, kt as functions of spin, inclination, distance
