@@ -16,6 +16,7 @@ bounded row sample of the same frame on the host cores.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -172,6 +173,64 @@ def batch_leg(args):
     }
 
 
+def grad_leg(args):
+    """The differentiable 512x512 scene: recording forward pass and adjoint
+    pass at the fit's start point against the target frame (L2 loss on
+    linear HDR), their HIP-event times and ratio, the plain FP64 trace of
+    the same frame, then an Adam fit from the start point."""
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.grad import GradRenderer, fit_adam
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, _ = K.preset("grad512")
+    out = {"workload": "512x512 observer scene, FP64, tol 1e-7, target (a 0.7, i 70deg, r 60M, mdot 1.0), "
+                       "start (0.5, 55deg, 70M, 0.8), checkpoint every 32 steps"}
+    with GradRenderer(cfg, checkpoint_interval=32) as g:
+        target, _ = g.forward(K.GRAD_TARGET)
+        target = target.clone()
+        fwd, bwd = [], []
+        for it in range(args.warmup + args.steps):
+            hdr, _ = g.forward(K.GRAD_START)
+            dl = 2.0 * (hdr - target)
+            dl[..., 3] = 0.0
+            g.backward(dl)
+            st = g.stats()
+            if it >= args.warmup:
+                fwd.append(st["forward_ms"])
+                bwd.append(st["backward_ms"])
+        tf, tb = sum(fwd) / len(fwd), sum(bwd) / len(bwd)
+        out.update({"forward_ms": round(tf, 4), "backward_ms": round(tb, 4),
+                    "backward_over_forward": round(tb / tf, 4), "log_chunks": st["chunks"],
+                    "forward_attempts": st["attempts"]})
+        # the same frame through the plain (non-recording) FP64 tracer
+        c2 = cfg.copy()
+        c2.spin, c2.mdot_scale, c2.escape_radius = K.GRAD_START[0], K.GRAD_START[3], K.GRAD_START[2]
+        cam = K.observer_camera(K.GRAD_START[2], math.degrees(K.GRAD_START[1]), K.GRAD_FOV_DEG)
+        with HipRenderer(c2) as r:
+            h2 = r.alloc_hdr()
+            r.trace(cam, hdr=h2, with_class=False)
+            plain = []
+            for _ in range(args.steps):
+                r.trace(cam, hdr=h2, with_class=False)
+                plain.append(r.stats()["kernel_event_ms"])
+        out["plain_forward_ms"] = round(sum(plain) / len(plain), 4)
+        out["backward_over_plain_forward"] = round(tb / out["plain_forward_ms"], 4)
+        if args.adam_iters > 0:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            theta, hist = fit_adam(g, target, K.GRAD_START, iters=args.adam_iters)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            losses = [h[0] for h in hist]
+            out["adam"] = {"iters": args.adam_iters, "seconds": round(wall, 3),
+                           "loss_first": losses[0], "loss_last": losses[-1], "loss_min": min(losses),
+                           "theta_final": [round(v, 6) for v in theta],
+                           "theta_target": [round(v, 6) for v in K.GRAD_TARGET],
+                           "loss_every_20": [float(f"{v:.6e}") for v in losses[::20]]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +240,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-render batch leg")
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-render leg")
+    ap.add_argument("--adam-iters", type=int, default=200, help="Adam iterations of the fit (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
@@ -271,6 +332,10 @@ def main():
     if not args.no_batch:
         print("[bench] batch leg", file=sys.stderr, flush=True)
         out["batch64"] = batch_leg(args)
+    if not args.no_grad:
+        print("[bench] gradient leg", file=sys.stderr, flush=True)
+        out["grad512"] = grad_leg(args)
+        out["backward_forward_ratio"] = out["grad512"]["backward_over_forward"]
 
     if not args.no_cpu:
         print("[bench] cpu baseline leg", file=sys.stderr, flush=True)

@@ -132,17 +132,20 @@ def render(theta, renderer):
     return _KERR_RENDER.apply(theta, renderer)
 
 
-def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), log_every=0):
+def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), final_lr=0.05, log_every=0):
     """Fit theta to a target frame with Adam on the L2 loss over the linear
     HDR colour channels.  Adam's step is about its learning rate in each
     coordinate whatever the gradient's scale, so theta is optimised as
-    theta = step * z with learning rate 1 on z: `step` is the per-iteration
-    step size in spin, inclination [rad], r_obs [M] and mdot.  Returns
-    (theta [4] list, history of (loss, theta) before each update)."""
+    theta = step * z on z: `step` is the initial per-iteration step size in
+    spin, inclination [rad], r_obs [M] and mdot, and the learning rate decays
+    geometrically from 1 to `final_lr` over the run.  Returns (theta [4]
+    list, history of (loss, theta) before each update)."""
     torch = _torch()
     scale = torch.tensor(step, dtype=torch.float64)
     z = (torch.tensor([float(v) for v in theta0], dtype=torch.float64) / scale).requires_grad_(True)
     opt = torch.optim.Adam([z], lr=1.0)
+    gamma = final_lr ** (1.0 / max(iters - 1, 1))
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=gamma)
     target = target_hdr[..., :3].detach().to(torch.float64)
     history = []
     for it in range(iters):
@@ -153,6 +156,7 @@ def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.0
         loss.backward()
         history.append((float(loss.detach()), [float(v) for v in theta.detach()]))
         opt.step()
+        sched.step()
         with torch.no_grad():
             z[0].clamp_(0.0, 0.999 / step[0])
             z[3].clamp_(min=1e-3 / step[3])
