@@ -80,9 +80,13 @@ struct Geo {
 
 // cuda/kerr_math.cuh:82-156: Hamiltonian right-hand side in Cartesian
 // Kerr-Schild coordinates, dx/dl = p - 2H (l.p - pt) l and
-// dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p).  Products that
-// feed a sum are fused with explicit fma; the reciprocals 1/s, 1/r,
-// 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division.
+// dp/dl = grad(H) (l.p - pt)^2 + 2H (l.p - pt) grad(l.p), with the position
+// derivatives in factored form: dr/dx_j = (x w1, y w2, z w1),
+// grad(l.p) = dr/dx_j G + (r px + a pz, py / r, r pz - a px) terms and
+// grad(H) = dr/dx_j HR + (0, HY, 0) (the reference's as-written dr/dy,
+// whose u/s term is halved, enters through w2).  Products that feed a sum
+// are fused with explicit fma; the reciprocals 1/s, 1/r, 1/(r^2+a^2),
+// 1/(r^4+a^2y^2) come from one division.
 KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
     const double a2 = g.aa, a = g.a;
@@ -102,16 +106,6 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     double inv_r = (s * dd) * q;
     double inv_denom = (sr * denomH) * q;
     double inv_denomH = (sr * denom) * q;
-    double dr2_dx = x * fma(u, inv_s, 1.0);
-    // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
-    // exact derivative of r^2 is y (1 + (u + 2a^2)/s)
-    double dr2_dy = g.exact ? y * fma(u + 2.0 * a2, inv_s, 1.0) : y * fma(u + 4.0 * a2, 0.5 * inv_s, 1.0);
-    double dr2_dz = z * fma(u, inv_s, 1.0);
-    double inv_2r = 0.5 * inv_r;
-    double drdx = dr2_dx * inv_2r;
-    double drdy = dr2_dy * inv_2r;
-    double drdz = dr2_dz * inv_2r;
-    double inv_denom2 = inv_denom * inv_denom;
     double Nx = fma(r, x, -(a * z));
     double Nz = fma(r, z, a * x);
     double lx = Nx * inv_denom;
@@ -125,42 +119,22 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     d.x = fma(-c, lx, px);
     d.y = fma(-c, ly, py);
     d.z = fma(-c, lz, pz);
-    double r_2 = 2.0 * r;
-    double dD_dx = r_2 * drdx;
-    double dD_dy = r_2 * drdy;
-    double dD_dz = r_2 * drdz;
-    double dNx_dx = fma(drdx, x, r);
-    double dNx_dy = drdy * x;
-    double dNx_dz = fma(drdz, x, -a);
-    double dNz_dx = fma(drdx, z, a);
-    double dNz_dy = drdy * z;
-    double dNz_dz = fma(drdz, z, r);
-    double dlx_dx = fma(dNx_dx, denom, -(Nx * dD_dx)) * inv_denom2;
-    double dlx_dy = fma(dNx_dy, denom, -(Nx * dD_dy)) * inv_denom2;
-    double dlx_dz = fma(dNx_dz, denom, -(Nx * dD_dz)) * inv_denom2;
-    double dlz_dx = fma(dNz_dx, denom, -(Nz * dD_dx)) * inv_denom2;
-    double dlz_dy = fma(dNz_dy, denom, -(Nz * dD_dy)) * inv_denom2;
-    double dlz_dz = fma(dNz_dz, denom, -(Nz * dD_dz)) * inv_denom2;
+    double inv_2r = 0.5 * inv_r;
+    double w1 = fma(u, inv_s, 1.0) * inv_2r;
+    double w2 = g.exact ? fma(u + 2.0 * a2, inv_s, 1.0) * inv_2r : fma(u + 4.0 * a2, 0.5 * inv_s, 1.0) * inv_2r;
+    double Q = fma(lx, px, lz * pz);
+    double XP = fma(x, px, z * pz);
     double inv_r2 = inv_r * inv_r;
-    double dly_dx = -(y * drdx) * inv_r2;
-    double dly_dy = fma(-y, drdy, r) * inv_r2;
-    double dly_dz = -(y * drdz) * inv_r2;
-    double r3x4 = 4.0 * r3;
-    double dDenH_dx = r3x4 * drdx;
-    double dDenH_dy = fma(r3x4, drdy, 2.0 * ya);
-    double dDenH_dz = r3x4 * drdz;
-    double factor_H = g.M * (inv_denomH * inv_denomH);
-    double t3 = (3.0 * r2) * denomH;
-    double dH_dx = factor_H * fma(t3, drdx, -(r3 * dDenH_dx));
-    double dH_dy = factor_H * fma(t3, drdy, -(r3 * dDenH_dy));
-    double dH_dz = factor_H * fma(t3, drdz, -(r3 * dDenH_dz));
-    double dlp_dx = fma(dlx_dx, px, fma(dly_dx, py, dlz_dx * pz));
-    double dlp_dy = fma(dlx_dy, px, fma(dly_dy, py, dlz_dy * pz));
-    double dlp_dz = fma(dlx_dz, px, fma(dly_dz, py, dlz_dz * pz));
+    double G = fma(fma(-2.0 * r, Q, XP), inv_denom, -((y * py) * inv_r2));
+    double inv_denomH2 = inv_denomH * inv_denomH;
+    double HR = ((g.M * r2) * inv_denomH2) * fma(3.0 * ya, y, -(r2 * r2));
+    double HY = -((2.0 * g.M) * ya) * (r3 * inv_denomH2);
     double lp2 = lp * lp;
-    d.px = fma(dH_dx, lp2, c * dlp_dx);
-    d.py = fma(dH_dy, lp2, c * dlp_dy);
-    d.pz = fma(dH_dz, lp2, c * dlp_dz);
+    double Z = fma(HR, lp2, c * G);
+    double cD = c * inv_denom;
+    d.px = fma(x * w1, Z, cD * fma(r, px, a * pz));
+    d.py = fma(y * w2, Z, fma(HY, lp2, (c * py) * inv_r));
+    d.pz = fma(z * w1, Z, cD * fma(r, pz, -(a * px)));
     return d;
 }
 

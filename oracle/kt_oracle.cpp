@@ -297,9 +297,21 @@ void ks_bl(S x, S y, S z, const Params<S>& p, S& r, S& sin_th, S& cos_th, S& phi
 // cuda/kerr_math.cuh:82-156 -- Hamiltonian right-hand side in Cartesian
 // Kerr-Schild coordinates: dx/dl = p - 2H(l.p - pt) l,
 // dp/dl = dH/dx (l.p - pt)^2 + 2H (l.p - pt) d(l.p)/dx.
-// Evaluation order (the device tracer repeats it operation for operation):
-// products that feed a sum are fused with explicit fma_, and the four
-// reciprocals 1/s, 1/r, 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division
+// The position derivatives are written in factored form.  With
+//   dr/dx = x w1, dr/dz = z w1, dr/dy = y w2,
+//   w1 = (1 + u/s) / 2r, w2 = (1 + (u + 2a^2)/s) / 2r
+// (as written the u/s term of dr^2/dy is halved, kerr_math.cuh:97:
+// w2 = (1 + (u + 4a^2)/2s) / 2r, used wherever dr/dy enters), the
+// derivatives of l and H the reference spells out component by component
+// (kerr_math.cuh:99-154) collapse to
+//   d(l.p)/dx_j = (dr/dx_j) G + [r px + a pz, py/r, r pz - a px]_j / D-terms,
+//   G = (x px + z pz - 2r (lx px + lz pz)) / (r^2+a^2) - y py / r^2,
+//   dH/dx_j = (dr/dx_j) HR + [0, HY, 0]_j,
+//   HR = M r^2 (3a^2y^2 - r^4) / (r^4+a^2y^2)^2,  HY = -2 M a^2 y r^3 / (r^4+a^2y^2)^2,
+// the same function in fewer operations.  Evaluation order (the device
+// tracer repeats it operation for operation): products that feed a sum are
+// fused with explicit fma_, and the four reciprocals 1/s, 1/r,
+// 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division
 // q = 1/(s r (r^2+a^2)(r^4+a^2y^2)).
 template <class S>
 State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
@@ -321,16 +333,6 @@ State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     S inv_r = (s * dd) * q;
     S inv_denom = (sr * denomH) * q;
     S inv_denomH = (sr * denom) * q;
-    S dr2_dx = x * fma_(u, inv_s, S(1.0));
-    // as written the u/s term of dr^2/dy is halved (kerr_math.cuh:97); the
-    // Hamiltonian mode uses the exact derivative y (1 + (u + 2a^2)/s)
-    S dr2_dy = p.rhs_exact ? y * fma_(u + 2.0 * a2, inv_s, S(1.0)) : y * fma_(u + 4.0 * a2, 0.5 * inv_s, S(1.0));
-    S dr2_dz = z * fma_(u, inv_s, S(1.0));
-    S inv_2r = 0.5 * inv_r;
-    S drdx = dr2_dx * inv_2r;
-    S drdy = dr2_dy * inv_2r;
-    S drdz = dr2_dz * inv_2r;
-    S inv_denom2 = inv_denom * inv_denom;
     S Nx = fma_(r, x, -(a * z));
     S Nz = fma_(r, z, a * x);
     S lx = Nx * inv_denom;
@@ -344,42 +346,22 @@ State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     d.x = fma_(-c, lx, px);
     d.y = fma_(-c, ly, py);
     d.z = fma_(-c, lz, pz);
-    S r_2 = 2.0 * r;
-    S dD_dx = r_2 * drdx;
-    S dD_dy = r_2 * drdy;
-    S dD_dz = r_2 * drdz;
-    S dNx_dx = fma_(drdx, x, r);
-    S dNx_dy = drdy * x;
-    S dNx_dz = fma_(drdz, x, -a);
-    S dNz_dx = fma_(drdx, z, a);
-    S dNz_dy = drdy * z;
-    S dNz_dz = fma_(drdz, z, r);
-    S dlx_dx = fma_(dNx_dx, denom, -(Nx * dD_dx)) * inv_denom2;
-    S dlx_dy = fma_(dNx_dy, denom, -(Nx * dD_dy)) * inv_denom2;
-    S dlx_dz = fma_(dNx_dz, denom, -(Nx * dD_dz)) * inv_denom2;
-    S dlz_dx = fma_(dNz_dx, denom, -(Nz * dD_dx)) * inv_denom2;
-    S dlz_dy = fma_(dNz_dy, denom, -(Nz * dD_dy)) * inv_denom2;
-    S dlz_dz = fma_(dNz_dz, denom, -(Nz * dD_dz)) * inv_denom2;
+    S inv_2r = 0.5 * inv_r;
+    S w1 = fma_(u, inv_s, S(1.0)) * inv_2r;
+    S w2 = p.rhs_exact ? fma_(u + 2.0 * a2, inv_s, S(1.0)) * inv_2r : fma_(u + 4.0 * a2, 0.5 * inv_s, S(1.0)) * inv_2r;
+    S Q = fma_(lx, px, lz * pz);
+    S XP = fma_(x, px, z * pz);
     S inv_r2 = inv_r * inv_r;
-    S dly_dx = -(y * drdx) * inv_r2;
-    S dly_dy = fma_(-y, drdy, r) * inv_r2;
-    S dly_dz = -(y * drdz) * inv_r2;
-    S r3x4 = 4.0 * r3;
-    S dDenH_dx = r3x4 * drdx;
-    S dDenH_dy = fma_(r3x4, drdy, 2.0 * ya);
-    S dDenH_dz = r3x4 * drdz;
-    S factor_H = p.M * (inv_denomH * inv_denomH);
-    S t3 = (3.0 * r2) * denomH;
-    S dH_dx = factor_H * fma_(t3, drdx, -(r3 * dDenH_dx));
-    S dH_dy = factor_H * fma_(t3, drdy, -(r3 * dDenH_dy));
-    S dH_dz = factor_H * fma_(t3, drdz, -(r3 * dDenH_dz));
-    S dlp_dx = fma_(dlx_dx, px, fma_(dly_dx, py, dlz_dx * pz));
-    S dlp_dy = fma_(dlx_dy, px, fma_(dly_dy, py, dlz_dy * pz));
-    S dlp_dz = fma_(dlx_dz, px, fma_(dly_dz, py, dlz_dz * pz));
+    S G = fma_(fma_(-2.0 * r, Q, XP), inv_denom, -((y * py) * inv_r2));
+    S inv_denomH2 = inv_denomH * inv_denomH;
+    S HR = ((p.M * r2) * inv_denomH2) * fma_(3.0 * ya, y, -(r2 * r2));
+    S HY = -((2.0 * p.M) * ya) * (r3 * inv_denomH2);
     S lp2 = lp * lp;
-    d.px = fma_(dH_dx, lp2, c * dlp_dx);
-    d.py = fma_(dH_dy, lp2, c * dlp_dy);
-    d.pz = fma_(dH_dz, lp2, c * dlp_dz);
+    S Z = fma_(HR, lp2, c * G);
+    S cD = c * inv_denom;
+    d.px = fma_(x * w1, Z, cD * fma_(r, px, a * pz));
+    d.py = fma_(y * w2, Z, fma_(HY, lp2, (c * py) * inv_r));
+    d.pz = fma_(z * w1, Z, cD * fma_(r, pz, -(a * px)));
     return d;
 }
 
