@@ -193,6 +193,8 @@ typedef struct kt_launch_opts {
     int32_t cost_order;     /* 0 or 1: visit tiles hardest-first (predicted
                                from the rays' closeness to the photon-orbit
                                critical curve); 2: row-major order */
+    int32_t long_ray_attempts; /* a wave holding a ray past this many attempts
+                               raises its issue priority (3x, 8x: higher) */
 } kt_launch_opts;
 
 typedef struct kt_renderer kt_renderer;

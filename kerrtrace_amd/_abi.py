@@ -136,6 +136,7 @@ class KtLaunchOpts(C.Structure):
         ("refill_threshold", C.c_int32),
         ("queue_chunk", C.c_int32),
         ("cost_order", C.c_int32),
+        ("long_ray_attempts", C.c_int32),
     ]
 
 

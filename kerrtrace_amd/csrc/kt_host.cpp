@@ -225,7 +225,7 @@ struct kt_renderer {
     hipEvent_t param_ev[kRing] = {};
     bool param_ev_used[kRing] = {};
     int param_next = 0;
-    kt_launch_opts opts{8, 8, 8, 1};
+    kt_launch_opts opts{8, 8, 8, 1, 256};
     hipStream_t last_stream = nullptr;
     uint64_t last_rays = 0;
     hipEvent_t ev_trace0 = nullptr, ev_trace1 = nullptr;  // around the tracer launch
@@ -376,6 +376,7 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     kt::RenderParams P;
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
     make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
+    P.long_ray = r->opts.long_ray_attempts > 0 ? (uint32_t)r->opts.long_ray_attempts : 256u;
     const bool order = r->opts.cost_order != 2 && tile <= 8;
     const int n_tiles = (int)P.n_tiles;
     const uint64_t items = (uint64_t)n_tiles * (uint64_t)tile * (uint64_t)tile;
@@ -441,7 +442,7 @@ struct kt_batch {
     uint32_t* d_hist = nullptr;
     uint32_t* d_perm = nullptr;
     size_t max_tiles = 0;
-    kt_launch_opts opts{8, 8, 8, 1};
+    kt_launch_opts opts{8, 8, 8, 1, 256};
     hipStream_t last_stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_params = nullptr;
     bool params_in_flight = false;
@@ -803,7 +804,7 @@ kt_status kt_get_luts(const kt_renderer* r, double* disk, double* bb) {
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* o) {
     if (!r || !o) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
-        o->queue_chunk < 0 || o->queue_chunk > 256 || o->cost_order < 0 || o->cost_order > 2)
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0)
         return set_err(r, KT_ERR_INVALID_ARGUMENT, "launch option out of range");
     r->opts = *o;
     return KT_OK;
@@ -1025,7 +1026,7 @@ void kt_batch_destroy(kt_batch* b) {
 kt_status kt_batch_set_launch_opts(kt_batch* b, const kt_launch_opts* o) {
     if (!b || !o) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
-        o->queue_chunk < 0 || o->queue_chunk > 256 || o->queue_chunk == 1 || o->cost_order < 0 || o->cost_order > 2)
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->queue_chunk == 1 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0)
         return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "launch option out of range (batch tile edge >= 2)");
     b->opts = *o;
     return KT_OK;
@@ -1046,6 +1047,7 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr, uint8_t* const* cls, vo
         if (!hdr[i]) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null HDR pointer");
         kt::RenderParams& P = b->h_params[i];
         make_params(nullptr, b->cfgs[i], b->ds[i], b->cams[i], hdr[i], cls ? cls[i] : nullptr, b->d_flag, tile, P);
+        P.long_ray = b->opts.long_ray_attempts > 0 ? (uint32_t)b->opts.long_ray_attempts : 256u;
         P.disk_lut = b->d_disk_luts[i];
         int bb = i;
         while (!b->d_bb_luts[bb]) --bb;  // the first render with the same table owns it
@@ -1237,6 +1239,7 @@ kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr, uint8_t
     kt::RenderParams& P = *g->h_params;
     const int tile = 8;
     make_params(nullptr, c, g->d, cam, hdr, g->d_cls, g->d_flag, tile, P);
+    P.long_ray = 256u;
     P.disk_lut = g->d_disk_lut;
     P.bb_lut = g->d_bb_lut;
     P.tile_perm = g->d_perm;

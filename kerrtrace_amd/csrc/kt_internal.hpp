@@ -59,7 +59,7 @@ struct RenderParams {
     uint32_t* error_flag;     // [0] set once, [1] float bits of the first overflowing T
     int32_t W, H, spp, max_steps, max_attempts, lut_n, noise_detail, disk_root_mode;
     int32_t rhs_exact;        // 1: exact Hamiltonian dr^2/dy
-    int32_t reserved;
+    uint32_t long_ray;        // attempts after which a ray raises its wave's priority
     // work queue: items are (global tile, pixel within the tile); the
     // tiles of all renders of a launch are numbered consecutively
     uint32_t tile_begin;      // first global tile of this render

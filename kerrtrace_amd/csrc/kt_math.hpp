@@ -58,14 +58,58 @@ KT_HDI double pow_m02(double x) {
     return ldexp(y * tj, -q);
 }
 
+// Correctly rounded FP64 square root and quotient for the operands the
+// tracer produces: positive, normal, exponents far from the extremes
+// (radii, squared radii, products of radii and step errors).  On the device
+// they are the refinement cores of the full library sequences without the
+// operand scaling and special-value fix-ups those wrap around them:
+//   sqrt: y = rsq(x), g = x y, h = y / 2, r = 1/2 - g h, g += g r,
+//         h += h r, then twice d = x - g^2, g += d h  (Goldschmidt);
+//   n / d: r = rcp(d), twice e = 1 - d r, r += r e; q = n r,
+//          q += (n - d q) r  (Markstein).
+// For such operands the library sequences run exactly these operations,
+// so the results are the same correctly rounded values; the host build
+// uses the library functions.
+KT_HDI double sqrt_pos(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return g;
+#else
+    return sqrt(x);
+#endif
+}
+KT_HDI double div_pos(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    const double q = n * r;
+    const double rem = fma(-d, q, n);
+    return fma(rem, r, q);
+#else
+    return n / d;
+#endif
+}
+
 // cuda/kerr_math.cuh:50-59: Kerr-Schild radius from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0
 // (s_sq * rsqrt(s_sq) evaluated as one square root).
 KT_HDI double ks_r(double x, double y, double z, double a2) {
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - a2;
     double s_sq = fma(u, u, 4.0 * (a2 * y) * y);
-    double r2 = 0.5 * (u + sqrt(s_sq));
-    return sqrt(r2);
+    double r2 = 0.5 * (u + sqrt_pos(s_sq));
+    return sqrt_pos(r2);
 }
 
 struct V6 {
@@ -94,14 +138,14 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     double u = rho2 - a2;
     double ya = a2 * y;
     double s_sq = fma(u, u, 4.0 * ya * y);
-    double s = sqrt(s_sq);
+    double s = sqrt_pos(s_sq);
     double r2 = 0.5 * (u + s);
-    double r = sqrt(r2);
+    double r = sqrt_pos(r2);
     double denom = r2 + a2;
     double denomH = fma(r2, r2, ya * y);
     double sr = s * r;
     double dd = denom * denomH;
-    double q = 1.0 / (sr * dd);
+    double q = div_pos(1.0, sr * dd);
     double inv_s = (r * dd) * q;
     double inv_r = (s * dd) * q;
     double inv_denom = (sr * denomH) * q;

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     K32 k1m{0, 0, 0, 0, 0, 0};  // FSAL stage-1 pieces of the current state
     LaneCounters cnt;
     bool drained = false;
-    bool high_prio = false;
+    int prio = 0;
+    const uint32_t long_ray = prm<BATCH>(Ps, 0).long_ray;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
     if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
@@ -115,15 +116,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         // a wave holding a long-running ray (photon-ring orbits) issues
         // ahead of the other wave(s) on its SIMD, so its serial chain of
         // steps, which bounds the frame time, is not slowed by them
-        if (__ballot(L.active && L.ray_att > kLongRay) != 0) {
-            if (!high_prio) {
-                __builtin_amdgcn_s_setprio(2);
-                high_prio = true;
-            }
-        } else if (high_prio) {
-            __builtin_amdgcn_s_setprio(0);
-            high_prio = false;
-        }
+        wave_priority(L, long_ray, prio);
         if (!L.active || L.finished) continue;
 
         // ---- one DP5(4) attempt, split form (see the file comment)

@@ -13,10 +13,16 @@
 // Its radial potential (Boyer-Lindquist, M = 1)
 //   R(r) = (r^2 + a^2 - a lambda)^2 - (r^2 - 2r + a^2)(eta + (lambda - a)^2)
 // has a double root exactly on the critical curve (spherical photon
-// orbits); the closer min_r |R(r)| / r^4 is to zero, the more orbits the
-// ray makes before it escapes or falls in.  key = -log of that minimum.
-// The key is evaluated in FP32: it only orders work, it never changes a
-// ray's arithmetic, so frames are identical with or without it.
+// orbits).  Near it R has a local minimum close to zero between r_+ and
+// the photon region: a ray just outside the curve turns back at a shallow
+// forbidden region, one just inside spirals over a shallow allowed one,
+// and either way it orbits ~ -log|R(r*)| times.  R is sampled on a
+// quadratic grid in [r_+, 12] (dense near the horizon, where the
+// prograde orbits of a rapidly spinning hole sit), the lowest sample is
+// refined with Newton steps on R' = 0, and key = -log(|R(r*)| / r*^4).
+// (The smallest |R| over samples would not do: |R| vanishes at every
+// simple turning point too.)  The key only orders work, it never changes
+// a ray's arithmetic, so frames are identical with or without it.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -46,40 +52,51 @@ __global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict_
     const int px = tx * T + lane % T, py = ty * T + lane / T;
     float key = -3.0e38f;
     if (lane < T * T && px < P.W && py < P.H) {
-        const float u = (float)((((double)px + 0.5) * P.inv_w_2 - 1.0) * P.aspect * P.fov_scale);
-        const float v = (float)((1.0 - ((double)py + 0.5) * P.inv_h_2) * P.fov_scale);
-        float n[3];
-        for (int c = 0; c < 3; ++c) n[c] = u * (float)P.rgt[c] + v * (float)P.up[c] + (float)P.fwd[c];
-        const float inv = rsqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const double u = (((double)px + 0.5) * P.inv_w_2 - 1.0) * P.aspect * P.fov_scale;
+        const double v = (1.0 - ((double)py + 0.5) * P.inv_h_2) * P.fov_scale;
+        double n[3];
+        for (int c = 0; c < 3; ++c) n[c] = u * P.rgt[c] + v * P.up[c] + P.fwd[c];
+        const double inv = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
         n[0] *= inv;
         n[1] *= inv;
         n[2] *= inv;
-        const float x = (float)P.cam[0], y = (float)P.cam[1], z = (float)P.cam[2];
-        const float jx = y * n[2] - z * n[1], jy = z * n[0] - x * n[2], jz = x * n[1] - y * n[0];
-        const float rc2 = x * x + y * y + z * z;
-        const float a = (float)(P.a * P.inv_M);  // in units of M
-        const float cos2 = y * y / rc2;
-        const float scale = (float)P.inv_M;
-        const float lam = jy * scale;
-        const float eta = (jx * jx + jz * jz) * scale * scale - a * a * cos2;
-        const float rp = 1.0f + sqrtf(fmaxf(1.0f - a * a, 0.0f));
-        const float c2 = eta + (lam - a) * (lam - a);
-        float best = 3.0e38f;
-        const int NS = 48;
+        const double x = P.cam[0], y = P.cam[1], z = P.cam[2];
+        const double jx = y * n[2] - z * n[1], jy = z * n[0] - x * n[2], jz = x * n[1] - y * n[0];
+        const double rc2 = x * x + y * y + z * z;
+        const double a = P.a * P.inv_M;  // in units of M
+        const double cos2 = y * y / rc2;
+        const double lam = jy * P.inv_M;
+        const double eta = (jx * jx + jz * jz) * (P.inv_M * P.inv_M) - a * a * cos2;
+        const double rp = 1.0 + sqrt(fmax(1.0 - a * a, 0.0));
+        const double c2 = eta + (lam - a) * (lam - a);
+        const double r_lo = rp * 1.0001, span = 12.0 - rp;
+        double best = 3.0e38, r_best = r_lo;
+        const int NS = 32;
         for (int i = 0; i < NS; ++i) {
-            const float r = rp * 1.0001f + (12.0f - rp) * ((float)i / (float)(NS - 1));
-            const float r2 = r * r;
-            const float D = r2 - 2.0f * r + a * a;
-            const float A = r2 + a * a - a * lam;
-            const float R = A * A - D * c2;
-            best = fminf(best, fabsf(R) / (r2 * r2));
+            const double t = (double)i / (double)(NS - 1);
+            const double r = r_lo + span * t * t;
+            const double A = r * r + a * a - a * lam;
+            const double R = A * A - (r * r - 2.0 * r + a * a) * c2;
+            if (R < best) {
+                best = R;
+                r_best = r;
+            }
         }
-        key = -__logf(best + 1e-12f);
+        double r = r_best;
+        for (int it = 0; it < 3; ++it) {
+            const double A = r * r + a * a - a * lam;
+            const double d1 = 4.0 * r * A - (2.0 * r - 2.0) * c2;
+            const double d2 = 4.0 * A + 8.0 * r * r - 2.0 * c2;
+            if (d2 > 0.0) r = fmin(fmax(r - d1 / d2, r_lo), 12.0);
+        }
+        const double A = r * r + a * a - a * lam;
+        const double R = fmin(A * A - (r * r - 2.0 * r + a * a) * c2, best);
+        key = (float)(-log(fabs(R) / (r * r * r * r) + 1e-12));
     }
     for (int off = 32; off > 0; off >>= 1) key = fmaxf(key, __shfl_xor(key, off));
     if (lane == 0) {
-        // -log spans about [-10, 28]; 6 levels per unit of log
-        int q = (int)((key + 10.0f) * 6.0f);
+        // -log spans about [-6, 20]; 8 levels per unit of log
+        int q = (int)((key + 10.0f) * 8.0f);
         keys[gtile] = (uint8_t)min(max(q, 0), kBuckets - 1);
     }
 }

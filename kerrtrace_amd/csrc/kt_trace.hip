@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     L.nrec = 0;
     LaneCounters cnt;
     bool drained = false;
-    bool high_prio = false;
+    int prio = 0;
+    const uint32_t long_ray = prm<BATCH>(Ps, 0).long_ray;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
     if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
@@ -73,15 +74,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         // a wave holding a long-running ray (photon-ring orbits) issues
         // ahead of the other wave(s) on its SIMD, so its serial chain of
         // steps, which bounds the frame time, is not slowed by them
-        if (__ballot(L.active && L.ray_att > kLongRay) != 0) {
-            if (!high_prio) {
-                __builtin_amdgcn_s_setprio(2);
-                high_prio = true;
-            }
-        } else if (high_prio) {
-            __builtin_amdgcn_s_setprio(0);
-            high_prio = false;
-        }
+        wave_priority(L, long_ray, prio);
         if (!L.active || L.finished) continue;
 
         // ---- one DP5(4) attempt (cuda/kerr_trace.cuh:83-122)
@@ -147,12 +140,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #undef KT_D7ACC
         // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
         const double tol = L.tol;
-#define KT_ERR(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
+#define KT_ERR(c) div_pos(fabs(hs * fma(KT_E7, k7.c, eacc.c)), tol * fmax(fabs(s.c), 1.0))
         double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
         double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
 #undef KT_ERR
 #undef KT_STAGE
         double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
+        if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
+            // a non-finite step error: the library quotients (special values)
+#define KT_ERRX(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
+            err = fmax(fmax(fmax(KT_ERRX(x), KT_ERRX(y)), fmax(KT_ERRX(z), KT_ERRX(px))),
+                       fmax(KT_ERRX(py), KT_ERRX(pz)));
+#undef KT_ERRX
+        }
         cnt.attempts++;
         L.ray_att++;
         L.att_in_step++;

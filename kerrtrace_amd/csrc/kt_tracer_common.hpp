@@ -40,8 +40,20 @@ namespace kt {
 #define KT_E7 (-1.0 / 40.0)
 
 static constexpr int kWave = 64;
-// attempts after which a ray counts as long-running (the frame mean is ~40-80)
-static constexpr uint32_t kLongRay = 256;
+
+struct Lane;
+// A wave holding long-running rays (photon-ring orbits) issues ahead of
+// the other wave(s) on its SIMD, so the serial chain of steps of the
+// frame's longest rays, which bounds the frame time, is slowed as little
+// as possible: priority 1 past `long_ray` attempts, 2 past 3x, 3 past 8x.
+__device__ __forceinline__ void set_wave_priority(int p) {
+    switch (p) {
+        case 0: __builtin_amdgcn_s_setprio(0); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        default: __builtin_amdgcn_s_setprio(3); break;
+    }
+}
 
 // Potential-scaled tolerance of the step that starts at s, whose
 // Kerr-Schild radius is r_c (cuda/kerr_trace.cuh:67-73, with the tolerance
@@ -383,6 +395,22 @@ __device__ __forceinline__ void end_sample(Lane& L, const RenderParams& P, const
         if (atomicCAS(P.error_flag, 0u, 1u) == 0u) P.error_flag[1] = __float_as_uint((float)L.overflow_T);
     }
     L.active = false;
+}
+
+__device__ __forceinline__ void wave_priority(const Lane& L, uint32_t long_ray, int& prio) {
+    const uint32_t att = L.active ? L.ray_att : 0u;
+    int want = 0;
+    if (__ballot(att > long_ray) != 0) {
+        want = 1;
+        if (__ballot(att > 3u * long_ray) != 0) {
+            want = 2;
+            if (__ballot(att > 8u * long_ray) != 0) want = 3;
+        }
+    }
+    if (want != prio) {
+        set_wave_priority(want);
+        prio = want;
+    }
 }
 
 // Recording traces: log committed step L.nrec (start state s, step size

@@ -67,8 +67,12 @@ class HipRenderer:
         return self.cfg.height
 
     # ---- stream-ordered device API
-    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0):
-        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order)
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0):
+        """Persistent-tracer knobs (0 = default): resident waves per CU, idle
+        lanes that trigger a refill, tile edge of the work order, tile order
+        (0/1 hardest-first, 2 row-major), attempts after which a ray raises
+        its wave's priority."""
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
         check(lib().kt_set_launch_opts(self._h, C.byref(o)), self._h)
 
     def alloc_hdr(self):
@@ -194,8 +198,8 @@ class HipBatch:
     def rays(self):
         return sum(c.width * c.height * c.spp for c in self.cfgs)
 
-    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0):
-        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order)
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0):
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
         check(lib().kt_batch_set_launch_opts(self._h, C.byref(o)))
 
     def alloc(self, with_class=True):
