@@ -202,66 +202,49 @@ __device__ __forceinline__ K32 rhs_mixed(double x, double y, double z, float pxf
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - aa_d;
     double s_sq = fma(u, u, 4.0 * aa_d * y * y);
-    double s = sqrt(s_sq);
+    double s = sqrt_pos(s_sq);
     double r2d = 0.5 * (u + s);
-    // FP32 bulk
+    // FP32 bulk, in the factored form of rhs()
     const float xf = (float)x, yf = (float)y, zf = (float)z;
-    const float a2 = g.aa;
+    const float a2 = g.aa, a = g.a;
     const float uf = (float)u;
     const float inv_s = __builtin_amdgcn_rcpf((float)s);
     const float r2 = (float)r2d;
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
-    float dr2_dx = xf * (1.0f + uf * inv_s);
-    float dr2_dy = g.exact ? yf * (1.0f + (uf + 2.0f * a2) * inv_s) : yf * (1.0f + (uf + 4.0f * a2) * (0.5f * inv_s));
-    float dr2_dz = zf * (1.0f + uf * inv_s);
-    float inv_2r = 0.5f * inv_r;
-    float drdx = dr2_dx * inv_2r, drdy = dr2_dy * inv_2r, drdz = dr2_dz * inv_2r;
-    float denom = r2 + a2;
-    float inv_denom = __builtin_amdgcn_rcpf(denom);
-    float inv_denom2 = inv_denom * inv_denom;
-    float Nx = r * xf - g.a * zf;
-    float Nz = r * zf + g.a * xf;
-    float lx = Nx * inv_denom;
-    float ly = yf * inv_r;
-    float lz = Nz * inv_denom;
-    float r3 = r2 * r;
-    float denomH = fmaf(r2, r2, a2 * yf * yf);
-    float inv_denomH = __builtin_amdgcn_rcpf(denomH);
-    float H = g.M * r3 * inv_denomH;
-    float lp = fmaf(lx, pxf, fmaf(ly, pyf, fmaf(lz, pzf, -ptf)));
-    float c = 2.0f * H * lp;
+    const float inv_denom = __builtin_amdgcn_rcpf(r2 + a2);
+    const float Nx = r * xf - a * zf;
+    const float Nz = r * zf + a * xf;
+    const float lx = Nx * inv_denom;
+    const float ly = yf * inv_r;
+    const float lz = Nz * inv_denom;
+    const float r3 = r2 * r;
+    const float ya2 = a2 * yf * yf;
+    const float inv_denomH = __builtin_amdgcn_rcpf(fmaf(r2, r2, ya2));
+    const float H = g.M * r3 * inv_denomH;
+    const float Q = fmaf(lx, pxf, lz * pzf);
+    const float lp = fmaf(ly, pyf, Q) - ptf;
+    const float c = 2.0f * H * lp;
     K32 k;
     k.cx = c * lx;
     k.cy = c * ly;
     k.cz = c * lz;
-    float dD_dx = 2.0f * r * drdx, dD_dy = 2.0f * r * drdy, dD_dz = 2.0f * r * drdz;
-    float dlx_dx = ((drdx * xf + r) * denom - Nx * dD_dx) * inv_denom2;
-    float dlx_dy = ((drdy * xf) * denom - Nx * dD_dy) * inv_denom2;
-    float dlx_dz = ((drdz * xf - g.a) * denom - Nx * dD_dz) * inv_denom2;
-    float dlz_dx = ((drdx * zf + g.a) * denom - Nz * dD_dx) * inv_denom2;
-    float dlz_dy = ((drdy * zf) * denom - Nz * dD_dy) * inv_denom2;
-    float dlz_dz = ((drdz * zf + r) * denom - Nz * dD_dz) * inv_denom2;
-    float inv_r2 = inv_r * inv_r;
-    float dly_dx = -yf * drdx * inv_r2;
-    float dly_dy = (r - yf * drdy) * inv_r2;
-    float dly_dz = -yf * drdz * inv_r2;
-    float f4r3 = 4.0f * r3;
-    float dDenH_dx = f4r3 * drdx;
-    float dDenH_dy = fmaf(f4r3, drdy, 2.0f * a2 * yf);
-    float dDenH_dz = f4r3 * drdz;
-    float factor_H = g.M * inv_denomH * inv_denomH;
-    float t3r2 = 3.0f * r2;
-    float dH_dx = factor_H * (t3r2 * drdx * denomH - r3 * dDenH_dx);
-    float dH_dy = factor_H * (t3r2 * drdy * denomH - r3 * dDenH_dy);
-    float dH_dz = factor_H * (t3r2 * drdz * denomH - r3 * dDenH_dz);
-    float dlp_dx = dlx_dx * pxf + dly_dx * pyf + dlz_dx * pzf;
-    float dlp_dy = dlx_dy * pxf + dly_dy * pyf + dlz_dy * pzf;
-    float dlp_dz = dlx_dz * pxf + dly_dz * pyf + dlz_dz * pzf;
-    float lp2 = lp * lp;
-    k.px = dH_dx * lp2 + c * dlp_dx;
-    k.py = dH_dy * lp2 + c * dlp_dy;
-    k.pz = dH_dz * lp2 + c * dlp_dz;
+    const float inv_2r = 0.5f * inv_r;
+    const float w1 = (1.0f + uf * inv_s) * inv_2r;
+    const float w2 = g.exact ? (1.0f + (uf + 2.0f * a2) * inv_s) * inv_2r
+                             : (1.0f + (uf + 4.0f * a2) * (0.5f * inv_s)) * inv_2r;
+    const float XP = fmaf(xf, pxf, zf * pzf);
+    const float inv_r2 = inv_r * inv_r;
+    const float G = fmaf(-2.0f * r, Q, XP) * inv_denom - yf * pyf * inv_r2;
+    const float inv_denomH2 = inv_denomH * inv_denomH;
+    const float HR = g.M * r2 * fmaf(3.0f, ya2, -(r2 * r2)) * inv_denomH2;
+    const float HY = -2.0f * g.M * a2 * yf * r3 * inv_denomH2;
+    const float lp2 = lp * lp;
+    const float Z = fmaf(HR, lp2, c * G);
+    const float cD = c * inv_denom;
+    k.px = fmaf(xf * w1, Z, cD * fmaf(r, pxf, a * pzf));
+    k.py = fmaf(yf * w2, Z, fmaf(HY, lp2, c * pyf * inv_r));
+    k.pz = fmaf(zf * w1, Z, cD * fmaf(r, pzf, -(a * pxf)));
     return k;
 }
 

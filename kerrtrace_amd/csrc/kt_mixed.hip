@@ -225,13 +225,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
                    fmaf(F(DD7), k7.px, dsum.px), fmaf(F(DD7), k7.py, dsum.py), fmaf(F(DD7), k7.pz, dsum.pz)};
         // embedded error: FP32 sums of the gravitational pieces, FP64 norm
         const double tol = L.tol;
-        double ex = fabs(h * (double)(hf * eq.x - esum.cx)) / (tol * fmax(fabs(X.x), 1.0));
-        double ey = fabs(h * (double)(hf * eq.y - esum.cy)) / (tol * fmax(fabs(X.y), 1.0));
-        double ez = fabs(h * (double)(hf * eq.z - esum.cz)) / (tol * fmax(fabs(X.z), 1.0));
-        double epx = fabs(h * (double)esum.px) / (tol * fmax(fabs(Pd.x), 1.0));
-        double epy = fabs(h * (double)esum.py) / (tol * fmax(fabs(Pd.y), 1.0));
-        double epz = fabs(h * (double)esum.pz) / (tol * fmax(fabs(Pd.z), 1.0));
+        double ex = div_pos(fabs(h * (double)(hf * eq.x - esum.cx)), tol * fmax(fabs(X.x), 1.0));
+        double ey = div_pos(fabs(h * (double)(hf * eq.y - esum.cy)), tol * fmax(fabs(X.y), 1.0));
+        double ez = div_pos(fabs(h * (double)(hf * eq.z - esum.cz)), tol * fmax(fabs(X.z), 1.0));
+        double epx = div_pos(fabs(h * (double)esum.px), tol * fmax(fabs(Pd.x), 1.0));
+        double epy = div_pos(fabs(h * (double)esum.py), tol * fmax(fabs(Pd.y), 1.0));
+        double epz = div_pos(fabs(h * (double)esum.pz), tol * fmax(fabs(Pd.z), 1.0));
         double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
+        if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
+            // a non-finite step error: the library quotients (special values)
+            err = fmax(fmax(fmax(fabs(h * (double)(hf * eq.x - esum.cx)) / (tol * fmax(fabs(X.x), 1.0)),
+                                 fabs(h * (double)(hf * eq.y - esum.cy)) / (tol * fmax(fabs(X.y), 1.0))),
+                            fmax(fabs(h * (double)(hf * eq.z - esum.cz)) / (tol * fmax(fabs(X.z), 1.0)),
+                                 fabs(h * (double)esum.px) / (tol * fmax(fabs(Pd.x), 1.0)))),
+                       fmax(fabs(h * (double)esum.py) / (tol * fmax(fabs(Pd.y), 1.0)),
+                            fabs(h * (double)esum.pz) / (tol * fmax(fabs(Pd.z), 1.0))));
+        }
         cnt.attempts++;
         L.ray_att++;
         L.att_in_step++;

@@ -1,8 +1,9 @@
 """Mixed-precision tracer (FP32 stage bulk, FP64 radius solve / error norm /
 controller / disk root / state) against the FP64 CPU oracle of the same
 frame.  Bar: hit classes differ only on boundary pixels and on at most
-0.1% of pixels; HDR relative error <= 1e-4 on pixels of equal class away
-from class boundaries."""
+0.1% of pixels; sub-classes (disk-contribution count, sky-grid line) differ
+only on boundary pixels and on at most 0.2% of pixels; HDR relative error
+<= 1e-4 on pixels of equal class away from class boundaries."""
 import sys
 
 import pytest
@@ -14,6 +15,7 @@ from tests.parity import compare
 pytestmark = pytest.mark.gpu
 
 MIXED_MAX_MISMATCH_FRAC = 1e-3
+MIXED_MAX_SUBCLASS_FRAC = 2e-3
 MIXED_MAX_REL = 1e-4
 
 
@@ -49,7 +51,8 @@ def test_mixed_matches_fp64_oracle(torch_gpu, name, size, tol):
     say(f"[mixed] {name}: {m}; attempts gpu {st['attempts']} oracle {ost.attempts}")
     assert m["class_mismatch_non_boundary"] <= 2
     assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
-    assert m["subclass_mismatch"] <= MIXED_MAX_MISMATCH_FRAC * m["pixels"]
+    assert m["subclass_mismatch_non_boundary"] <= 2
+    assert m["subclass_mismatch"] <= MIXED_MAX_SUBCLASS_FRAC * m["pixels"]
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
     # the split stage form keeps FP32 noise out of the error norm: the step
     # count stays close to the FP64 integration's
@@ -68,5 +71,6 @@ def test_mixed_spins_inclination_60(torch_gpu, spin):
     m = compare(g_hdr, g_cls, o_hdr, o_cls)
     say(f"[mixed] spin {spin}: {m}")
     assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
-    assert m["subclass_mismatch"] <= MIXED_MAX_MISMATCH_FRAC * m["pixels"]
+    assert m["subclass_mismatch_non_boundary"] <= 2
+    assert m["subclass_mismatch"] <= MIXED_MAX_SUBCLASS_FRAC * m["pixels"]
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
