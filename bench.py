@@ -29,6 +29,17 @@ PEAK_FP32_VECTOR = 157.3e12  # MI355X FP32 vector FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
 
 
+def pmc_traffic():
+    """HBM bytes per tracer launch measured with rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE on this workload (profiles/, with the commands that made
+    them), or None."""
+    path = os.path.join(ROOT, "profiles", "trace_fp64_hd1080_a0998_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def flop_constants():
     with open(os.path.join(ROOT, "tests", "golden", "flop_constants.json")) as fh:
         return json.load(fh)
@@ -122,19 +133,24 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
     return out
 
 
-def batch_leg(args):
-    """The 64-render seed-7 batch (kerrtrace_amd/presets/batch64_seed7.json):
+def batch_leg(args, precision, tol):
+    """The 64-render seed-7 batch (kerrtrace_amd/presets/batch64_seed7.json)
+    at one precision and tolerance: launch options from the autotuner, then
     one persistent launch over every render's tiles vs 64 launches of one
-    render each, both timed with HIP events around the tracer launches."""
+    render each (same options), both timed with HIP events around the
+    tracer launches."""
     import torch
 
     from kerrtrace_amd import config as K
     from kerrtrace_amd.renderer import HipBatch, HipRenderer
     renders = K.batch_preset()
+    for c, _ in renders:
+        c.precision = precision
+        c.tolerance = tol
     rays = sum(c.width * c.height * c.spp for c, _ in renders)
     b = HipBatch(renders)
     hdr, _ = b.alloc(with_class=False)
-    b.trace(hdr=hdr)
+    opts = b.autotune(hdr=hdr)
     torch.cuda.synchronize()
     t_batch = []
     for _ in range(args.steps):
@@ -143,6 +159,7 @@ def batch_leg(args):
         t_batch.append(st["kernel_event_ms"])
     singles = [HipRenderer(c) for c, _ in renders]
     for r, (_, cam), h in zip(singles, renders, hdr):
+        r.set_launch_opts(**opts)
         r.trace(cam, hdr=h, with_class=False)
     torch.cuda.synchronize()
     t_seq = []
@@ -160,8 +177,11 @@ def batch_leg(args):
     tb = sum(t_batch) / len(t_batch)
     ts = sum(t_seq) / len(t_seq)
     return {
+        "precision": "mixed" if precision == K.KT_PRECISION_MIXED else "fp64",
+        "tolerance": tol,
         "renders": len(renders),
         "rays": rays,
+        "launch_opts": opts,
         "batched_ms": round(tb, 4),
         "batched_mrays_s": round(rays / tb / 1e3, 4),
         "sequential_ms": round(ts, 4),
@@ -215,6 +235,11 @@ def grad_leg(args):
                 r.trace(cam, hdr=h2, with_class=False)
                 plain.append(r.stats()["kernel_event_ms"])
         out["plain_forward_ms"] = round(sum(plain) / len(plain), 4)
+        from kerrtrace_amd.grad import autotune_checkpoint
+        best_k, times = autotune_checkpoint(cfg, K.GRAD_START)
+        out["checkpoint_autotune"] = {"best_interval": best_k,
+                                      "backward_ms": {str(k): round(v, 4) for k, v in times.items()},
+                                      "backward_over_forward": round(times[best_k] / tf, 4)}
         out["backward_over_plain_forward"] = round(tb / out["plain_forward_ms"], 4)
         if args.adam_iters > 0:
             torch.cuda.synchronize()
@@ -241,6 +266,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-render batch leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-render leg")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch options for the 1080p frame")
     ap.add_argument("--adam-iters", type=int, default=200, help="Adam iterations of the fit (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -261,6 +287,7 @@ def main():
     hdr = r.alloc_hdr()
     bloom = torch.empty_like(hdr)
     ldr = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    launch_opts = None if args.no_autotune else r.autotune(cam, hdr=hdr)
     for _ in range(args.warmup):
         r.trace(cam, hdr=hdr, with_class=False)
         r.post_process(hdr, ldr=ldr, bloom=bloom)
@@ -292,6 +319,7 @@ def main():
     t_kernel = sum(kernel_ms) / len(kernel_ms) / 1e3
     t_post = sum(post_ms) / len(post_ms) / 1e3
     flops = trace_flops(st, fc)
+    traffic = pmc_traffic()
     achieved = flops / t_kernel
     post_bytes = 20.0 * W * H  # compulsory: float4 HDR read + u32 write per pixel
     out = {
@@ -308,7 +336,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "1920x1080 a=0.998 i=85deg r_obs=100M fov=20deg tol=1e-9 FP64 trace + bloom(sigma 12) + ACES/sRGB",
-                   "rays": rays, "parallelism": "single GPU"},
+                   "rays": rays, "parallelism": "single GPU", "launch_opts": launch_opts},
         "trace_mrays_s": round(rays / t_trace / 1e6, 4),
         "trace_ms": round(t_trace * 1e3, 4),
         "trace_kernel_ms": round(t_kernel * 1e3, 4),
@@ -319,7 +347,8 @@ def main():
                  "attempts_per_ray": round(st["attempts"] / rays, 3),
                  "kernel_ms": round(st["kernel_us"] / 1e3, 4), "tail_ms": round(st["tail_us"] / 1e3, 4)},
         "roofline": {"bound": "valu-fp64", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4),
+                     "traffic": (traffic or {}).get("bytes_per_launch"),
                      "kernel": "trace_fp64_persistent",
                      "flop_per_frame": flops},
         "post_roofline": {"bound": "hbm", "achieved": round(post_bytes / t_post / 1e9, 2), "peak": PEAK_HBM / 1e9,
@@ -331,7 +360,8 @@ def main():
         out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_kernel)
     if not args.no_batch:
         print("[bench] batch leg", file=sys.stderr, flush=True)
-        out["batch64"] = batch_leg(args)
+        out["batch64"] = {"fp64_tol_1e-9": batch_leg(args, K.KT_PRECISION_FP64, 1e-9),
+                          "mixed_tol_1e-7": batch_leg(args, K.KT_PRECISION_MIXED, 1e-7)}
     if not args.no_grad:
         print("[bench] gradient leg", file=sys.stderr, flush=True)
         out["grad512"] = grad_leg(args)

@@ -270,6 +270,19 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr,
                          uint8_t* const* hit_class, void* stream);
 kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out);
 
+/* ---- autotuner (new) ---------------------------------------------------
+ * Traces the frame (kt_autotune: camera cam into the device buffer hdr;
+ * kt_batch_autotune: every render into hdr[i]) with each of a fixed set
+ * of launch options -- waves per CU, refill threshold (rays a wave keeps
+ * before it takes new work), work-order tile edge -- keeps the fastest
+ * (best of two runs, HIP events around the trace) as the handle's launch
+ * options and returns it in *best (may be NULL).  The buffers hold the
+ * last trial's frame afterwards.  Synchronises the stream. */
+kt_status kt_autotune(kt_renderer* r, const kt_camera* cam, float* hdr,
+                      void* stream, kt_launch_opts* best);
+kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream,
+                            kt_launch_opts* best);
+
 /* ---- differentiable renderer (new: the reference has no gradients) -----
  * The observer scene as a function of theta = {spin a, inclination i [rad],
  * observer distance r_obs [M], accretion-rate scale mdot}: camera at

@@ -40,6 +40,8 @@ SIGNATURES = {
     "kt_batch_set_launch_opts": ([_vp, C.POINTER(KtLaunchOpts)], _i32),
     "kt_batch_trace": ([_vp, C.POINTER(_vp), C.POINTER(_vp), _vp], _i32),
     "kt_batch_get_stats": ([_vp, C.POINTER(KtTraceStats)], _i32),
+    "kt_autotune": ([_vp, C.POINTER(KtCamera), _vp, _vp, C.POINTER(KtLaunchOpts)], _i32),
+    "kt_batch_autotune": ([_vp, C.POINTER(_vp), _vp, C.POINTER(KtLaunchOpts)], _i32),
     "kt_grad_create": ([C.POINTER(KtConfig), C.POINTER(KtGradOpts), C.POINTER(_vp)], _i32),
     "kt_grad_destroy": ([_vp], None),
     "kt_grad_forward": ([_vp, _pd, _vp, _vp, _vp], _i32),

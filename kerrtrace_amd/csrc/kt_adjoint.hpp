@@ -94,68 +94,61 @@ KT_HDI void v6_add(V6& acc, const V6& x) {
 
 // --------------------------------------------------- right-hand side VJP
 // lamY += w^T df/dY at stage input Y; lam_pt += w . df/dpt;
-// lam_a += w . df/da (exact right-hand side only).
+// lam_a += w . df/da (exact right-hand side only).  The dual pass runs the
+// factored right-hand side of rhs() (one reciprocal for 1/s, 1/r,
+// 1/(r^2+a^2), 1/(r^4+a^2y^2)); only its tangents are used.
+KT_HDI D1 drcp(D1 b) {
+    const double iv = div_pos(1.0, b.v);
+    return D1{iv, -(b.d * iv) * iv};
+}
+KT_HDI D1 dsqrt_pos(D1 a) {
+    const double r = sqrt_pos(a.v);
+    return D1{r, a.d * (0.5 * div_pos(1.0, r))};
+}
 KT_HDI void rhs_vjp(const V6& Y, double pt, const Geo& g, const V6& w, V6& lamY, double& lam_pt, double& lam_a) {
     const D1 x{Y.x, -w.px}, y{Y.y, -w.py}, z{Y.z, -w.pz};
     const D1 px{Y.px, w.x}, py{Y.py, w.y}, pz{Y.pz, w.z};
     const double a2 = g.aa, a = g.a, M = g.M;
-    D1 rho2 = dfma(x, x, dfma(y, y, z * z));
-    D1 u = rho2 - a2;
-    D1 ya = a2 * y;
-    D1 s_sq = dfma(u, u, 4.0 * ya * y);
-    D1 s = dsqrt(s_sq);
-    D1 r2 = 0.5 * (u + s);
-    D1 r = dsqrt(r2);
-    D1 denom = r2 + a2;
-    D1 denomH = dfma(r2, r2, ya * y);
-    D1 inv_s = 1.0 / s;
-    D1 inv_r = 1.0 / r;
-    D1 inv_denom = 1.0 / denom;
-    D1 inv_denomH = 1.0 / denomH;
-    D1 dr2_dx = x * (u * inv_s + 1.0);
-    D1 dr2_dy = y * ((u + 2.0 * a2) * inv_s + 1.0);
-    D1 dr2_dz = z * (u * inv_s + 1.0);
-    D1 inv_2r = 0.5 * inv_r;
-    D1 drdx = dr2_dx * inv_2r, drdy = dr2_dy * inv_2r, drdz = dr2_dz * inv_2r;
-    D1 inv_denom2 = inv_denom * inv_denom;
-    D1 Nx = r * x - a * z;
-    D1 Nz = r * z + a * x;
-    D1 lx = Nx * inv_denom, ly = y * inv_r, lz = Nz * inv_denom;
-    D1 r3 = r2 * r;
-    D1 H = (M * r3) * inv_denomH;
-    D1 lp = dfma(lx, px, dfma(ly, py, dfma(lz, pz, cst(-pt))));
-    D1 c = (2.0 * H) * lp;
-    D1 fx = px - c * lx, fy = py - c * ly, fz = pz - c * lz;
-    D1 r_2 = 2.0 * r;
-    D1 dD_dx = r_2 * drdx, dD_dy = r_2 * drdy, dD_dz = r_2 * drdz;
-    D1 dNx_dx = drdx * x + r, dNx_dy = drdy * x, dNx_dz = drdz * x - a;
-    D1 dNz_dx = drdx * z + a, dNz_dy = drdy * z, dNz_dz = drdz * z + r;
-    D1 dlx_dx = (dNx_dx * denom - Nx * dD_dx) * inv_denom2;
-    D1 dlx_dy = (dNx_dy * denom - Nx * dD_dy) * inv_denom2;
-    D1 dlx_dz = (dNx_dz * denom - Nx * dD_dz) * inv_denom2;
-    D1 dlz_dx = (dNz_dx * denom - Nz * dD_dx) * inv_denom2;
-    D1 dlz_dy = (dNz_dy * denom - Nz * dD_dy) * inv_denom2;
-    D1 dlz_dz = (dNz_dz * denom - Nz * dD_dz) * inv_denom2;
-    D1 inv_r2 = inv_r * inv_r;
-    D1 dly_dx = -(y * drdx) * inv_r2;
-    D1 dly_dy = (r - y * drdy) * inv_r2;
-    D1 dly_dz = -(y * drdz) * inv_r2;
-    D1 r3x4 = 4.0 * r3;
-    D1 dDenH_dx = r3x4 * drdx;
-    D1 dDenH_dy = r3x4 * drdy + 2.0 * ya;
-    D1 dDenH_dz = r3x4 * drdz;
-    D1 factor_H = M * (inv_denomH * inv_denomH);
-    D1 t3 = (3.0 * r2) * denomH;
-    D1 dH_dx = factor_H * (t3 * drdx - r3 * dDenH_dx);
-    D1 dH_dy = factor_H * (t3 * drdy - r3 * dDenH_dy);
-    D1 dH_dz = factor_H * (t3 * drdz - r3 * dDenH_dz);
-    D1 dlp_dx = dlx_dx * px + dly_dx * py + dlz_dx * pz;
-    D1 dlp_dy = dlx_dy * px + dly_dy * py + dlz_dy * pz;
-    D1 dlp_dz = dlx_dz * px + dly_dz * py + dlz_dz * pz;
-    D1 lp2 = lp * lp;
-    D1 fpx = dH_dx * lp2 + c * dlp_dx;
-    D1 fpy = dH_dy * lp2 + c * dlp_dy;
-    D1 fpz = dH_dz * lp2 + c * dlp_dz;
+    const D1 rho2 = dfma(x, x, dfma(y, y, z * z));
+    const D1 u = rho2 - a2;
+    const D1 ya = a2 * y;
+    const D1 yy = y * y;
+    const D1 s = dsqrt_pos(dfma(u, u, 4.0 * (ya * y)));
+    const D1 r2 = 0.5 * (u + s);
+    const D1 r = dsqrt_pos(r2);
+    const D1 denom = r2 + a2;
+    const D1 denomH = dfma(r2, r2, ya * y);
+    const D1 sr = s * r;
+    const D1 dd = denom * denomH;
+    const D1 q = drcp(sr * dd);
+    const D1 inv_s = (r * dd) * q;
+    const D1 inv_r = (s * dd) * q;
+    const D1 inv_denom = (sr * denomH) * q;
+    const D1 inv_denomH = (sr * denom) * q;
+    const D1 Nx = dfma(r, x, -(a * z));
+    const D1 Nz = dfma(r, z, a * x);
+    const D1 lx = Nx * inv_denom, ly = y * inv_r, lz = Nz * inv_denom;
+    const D1 r3 = r2 * r;
+    const D1 H = (M * r3) * inv_denomH;
+    const D1 Q = dfma(lx, px, lz * pz);
+    const D1 lp = dfma(ly, py, Q) - pt;
+    const D1 c = (2.0 * H) * lp;
+    const D1 fx = px - c * lx, fy = py - c * ly, fz = pz - c * lz;
+    const D1 inv_2r = 0.5 * inv_r;
+    const D1 w1 = (u * inv_s + 1.0) * inv_2r;
+    const D1 w2 = ((u + 2.0 * a2) * inv_s + 1.0) * inv_2r;
+    const D1 XP = dfma(x, px, z * pz);
+    const D1 inv_r2 = inv_r * inv_r;
+    const D1 G = dfma(-2.0 * r, Q, XP) * inv_denom - (y * py) * inv_r2;
+    const D1 inv_denomH2 = inv_denomH * inv_denomH;
+    const D1 HR = ((M * r2) * inv_denomH2) * (3.0 * (ya * y) - r2 * r2);
+    const D1 HY = -(2.0 * M) * ya * (r3 * inv_denomH2);
+    const D1 lp2 = lp * lp;
+    const D1 Z = dfma(HR, lp2, c * G);
+    const D1 cD = c * inv_denom;
+    const D1 fpx = dfma(x * w1, Z, cD * dfma(r, px, a * pz));
+    const D1 fpy = dfma(y * w2, Z, dfma(HY, lp2, (c * py) * inv_r));
+    const D1 fpz = dfma(z * w1, Z, cD * dfma(r, pz, -(a * px)));
     // w^T df/dY = J^T (tangent of f along v)
     lamY.x -= fpx.d;
     lamY.y -= fpy.d;
@@ -164,17 +157,18 @@ KT_HDI void rhs_vjp(const V6& Y, double pt, const Geo& g, const V6& w, V6& lamY,
     lamY.py += fy.d;
     lamY.pz += fz.d;
     // parameters: pt enters Ham through lp; a through r, l and H
-    lam_pt += (2.0 * H * lp).d;
-    D1 r2_a = a * ((2.0 * y * y - u) * inv_s - 1.0);
-    D1 r_a = r2_a * inv_2r;
-    D1 D_a = r2_a + 2.0 * a;
-    D1 lx_a = ((r_a * x - z) * denom - Nx * D_a) * inv_denom2;
-    D1 lz_a = ((r_a * z + x) * denom - Nz * D_a) * inv_denom2;
-    D1 ly_a = -(y * r_a) * inv_r2;
-    D1 DH_a = 2.0 * r2 * r2_a + (2.0 * a) * (y * y);
-    D1 H_a = factor_H * (t3 * r_a - r3 * DH_a);
-    D1 lp_a = lx_a * px + ly_a * py + lz_a * pz;
-    D1 ham_a = -(H_a * lp2 + 2.0 * H * lp * lp_a);
+    lam_pt += 2.0 * fma(H.d, lp.v, H.v * lp.d);
+    const D1 r2_a = a * ((2.0 * yy - u) * inv_s - 1.0);
+    const D1 r_a = r2_a * inv_2r;
+    const D1 D_a = r2_a + 2.0 * a;
+    const D1 inv_denom2 = inv_denom * inv_denom;
+    const D1 lx_a = ((r_a * x - z) * denom - Nx * D_a) * inv_denom2;
+    const D1 lz_a = ((r_a * z + x) * denom - Nz * D_a) * inv_denom2;
+    const D1 ly_a = -(y * r_a) * inv_r2;
+    const D1 DH_a = 2.0 * r2 * r2_a + (2.0 * a) * yy;
+    const D1 H_a = (M * inv_denomH2) * ((3.0 * r2) * denomH * r_a - r3 * DH_a);
+    const D1 lp_a = dfma(lx_a, px, dfma(ly_a, py, lz_a * pz));
+    const D1 ham_a = -(H_a * lp2 + 2.0 * H * lp * lp_a);
     lam_a += ham_a.d;
 }
 

@@ -28,9 +28,10 @@
 
 namespace kt {
 
-// Per-lane scratch: element j of lane `tid` at sc[j * stride]; the first
-// 6 K elements hold the segment's states, the next 30 the stage inputs of
-// the step being reversed.
+// Per-lane scratch: element j of lane `tid` at sc[j * stride].  Step u of
+// the segment being reversed holds its start state at j = 36 u and the
+// inputs of its stages 2..6 at j = 36 u + 6 .. 36 u + 35; the state after
+// the segment's last step sits at j = 36 K.
 struct LaneScratch {
     double* p;
     uint32_t stride;
@@ -47,6 +48,49 @@ struct LaneScratch {
                   p[(size_t)(j + 3) * stride], p[(size_t)(j + 4) * stride], p[(size_t)(j + 5) * stride]};
     }
 };
+
+// Adjoint of a step's equatorial crossing (kept out of line: crossings are
+// about one step in a hundred).  Given the adjoint of the transmittance
+// after the step, it returns the one before it and the direct adjoints of
+// the step's start and end states, of k1, k3..k7 and of pt, kt, spin, mdot.
+struct CrossAdj {
+    V6 lk[6];
+    V6 lam_s, lam_ns;
+    double lamTr, lam_pt, lam_kt, g_a, g_m;
+    bool lit;
+};
+static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& P, const GradConsts& G, V6 sn, V6 ns,
+                                                         V6 y3, V6 y4, V6 y5, V6 y6, double hs, double pt,
+                                                         double kt, double Tr_n, double W0, double W1, double W2,
+                                                         double lamTr) {
+    CrossAdj o;
+    o.lit = false;
+    const Geo g{P.a, P.aa, P.M, true};
+    // the step's dense-output ingredients, as the forward pass formed them
+    const V6 k1 = rhs(sn, pt, g), k3 = rhs(y3, pt, g), k4 = rhs(y4, pt, g);
+    const V6 k5 = rhs(y5, pt, g), k6 = rhs(y6, pt, g), k7 = rhs(ns, pt, g);
+#define KT_DS(c) ((Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c) + Dense::d6 * k6.c) + \
+                     Dense::d7 * k7.c
+    const V6 dsum{KT_DS(x), KT_DS(y), KT_DS(z), KT_DS(px), KT_DS(py), KT_DS(pz)};
+#undef KT_DS
+    const Crossing hit = crossing_hit(sn, ns, k1, k7, dsum, hs, P.disk_root_mode);
+    const ShadeD sv = shade_value(P, G, hit, pt, kt);
+    if (!sv.lit) return o;
+    // col += c Tr_n;  Tr_{n+1} = Tr_n (1 - alpha)
+    const double c0 = sv.c[0].v, c1 = sv.c[1].v, c2 = sv.c[2].v, al = sv.alpha.v;
+    const double lam_c[3] = {W0 * Tr_n, W1 * Tr_n, W2 * Tr_n};
+    const double lam_alpha = -lamTr * Tr_n;
+    o.lamTr = W0 * c0 + W1 * c1 + W2 * c2 + lamTr * (1.0 - al);
+    o.lam_pt = o.lam_kt = o.g_a = o.g_m = 0.0;
+    double lam_hit[5];
+    shade_reverse(P, G, hit, pt, kt, lam_c, lam_alpha, lam_hit, o.lam_pt, o.lam_kt, o.g_a, o.g_m);
+    for (int i = 0; i < 6; ++i) o.lk[i] = v6_zero();
+    o.lam_s = v6_zero();
+    o.lam_ns = v6_zero();
+    crossing_reverse(sn, ns, k1, k7, dsum, hs, P.disk_root_mode, hit.t, lam_hit, o.lam_s, o.lam_ns, o.lk);
+    o.lit = true;
+    return o;
+}
 
 // One pixel's gradient contribution: out = d(sum_c w_c hdr_c)/d theta.
 __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, int py, const double w[3],
@@ -69,24 +113,26 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
     uint32_t chunk = LG.head[pix];
     if (chunk >= LG.capacity) return;
     uint32_t seg_start = ((N - 1) / K) * K;
-    const uint32_t jY = 6 * K;
+    const uint32_t j_end = 36 * K;
     for (;;) {
         const double* ch = LG.arena + (size_t)chunk * LG.stride;
         const uint32_t len = min(K, N - seg_start);
+        // re-integrate the segment from its checkpoint with the logged steps
         V6 s{ch[0], ch[1], ch[2], ch[3], ch[4], ch[5]};
-        sc.put6(0, s);
-        for (uint32_t u = 1; u < len; ++u) {
-            s = dp_step(s, ch[8 + u - 1], pt, g, nullptr).ns;
-            sc.put6(6 * u, s);
+        for (uint32_t u = 0; u < len; ++u) {
+            V6 Y[5];
+            sc.put6(36 * u, s);
+            s = dp_step(s, ch[8 + u], pt, g, Y).ns;
+            for (int i = 0; i < 5; ++i) sc.put6(36 * u + 6 + 6 * i, Y[i]);
         }
+        sc.put6(j_end, s);
         for (int u = (int)len - 1; u >= 0; --u) {
-            const V6 sn = sc.get6(6 * (uint32_t)u);
+            const V6 sn = sc.get6(36 * (uint32_t)u);
+            const V6 ns = sc.get6((uint32_t)u + 1 < len ? 36 * ((uint32_t)u + 1) : j_end);
             const double hs = ch[8 + u];
             const double Tr_n = ch[8 + K + u];
             V6 Y[5];
-            const StepOut so = dp_step(sn, hs, pt, g, Y);
-            for (int i = 0; i < 5; ++i) sc.put6(jY + 6 * i, Y[i]);
-            const V6 ns = so.ns;
+            for (int i = 0; i < 5; ++i) Y[i] = sc.get6(36 * (uint32_t)u + 6 + 6 * i);
             const double r_now = ks_r(ns.x, ns.y, ns.z, P.aa);
             const bool horizon = r_now < P.rh_eps;
             const bool crossing = !horizon && ((sn.y < 0.0 && ns.y >= 0.0) || (sn.y > 0.0 && ns.y <= 0.0));
@@ -94,33 +140,20 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
             V6 lam_s_dir = v6_zero();
             bool have_lk = false;
             if (crossing) {
-                const V6 k7 = rhs(ns, pt, g);
-                V6 dsum = so.dacc6;
-                dsum.x = dsum.x + Dense::d7 * k7.x;
-                dsum.y = dsum.y + Dense::d7 * k7.y;
-                dsum.z = dsum.z + Dense::d7 * k7.z;
-                dsum.px = dsum.px + Dense::d7 * k7.px;
-                dsum.py = dsum.py + Dense::d7 * k7.py;
-                dsum.pz = dsum.pz + Dense::d7 * k7.pz;
-                const Crossing hit = crossing_hit(sn, ns, so.k1, k7, dsum, hs, P.disk_root_mode);
-                const ShadeD sv = shade_value(P, G, hit, pt, kt);
-                if (sv.lit) {
-                    // col += c Tr_n;  Tr_{n+1} = Tr_n (1 - alpha)
-                    const double c0 = sv.c[0].v, c1 = sv.c[1].v, c2 = sv.c[2].v, al = sv.alpha.v;
-                    const double lam_c[3] = {W0 * Tr_n, W1 * Tr_n, W2 * Tr_n};
-                    const double lam_alpha = -lamTr * Tr_n;
-                    lamTr = W0 * c0 + W1 * c1 + W2 * c2 + lamTr * (1.0 - al);
-                    double lam_hit[5];
-                    shade_reverse(P, G, hit, pt, kt, lam_c, lam_alpha, lam_hit, lam_pt, lam_kt, g_a, g_m);
-                    for (int i = 0; i < 6; ++i) lk[i] = v6_zero();
-                    V6 lam_ns_dir = v6_zero();
-                    crossing_reverse(sn, ns, so.k1, k7, dsum, hs, P.disk_root_mode, hit.t, lam_hit, lam_s_dir,
-                                     lam_ns_dir, lk);
-                    v6_add(lam, lam_ns_dir);
+                CrossAdj ca = crossing_adjoint(P, G, sn, ns, Y[1], Y[2], Y[3], Y[4], hs, pt, kt, Tr_n, W0, W1, W2,
+                                               lamTr);
+                if (ca.lit) {
+                    lamTr = ca.lamTr;
+                    lam_pt += ca.lam_pt;
+                    lam_kt += ca.lam_kt;
+                    g_a += ca.g_a;
+                    g_m += ca.g_m;
+                    for (int i = 0; i < 6; ++i) lk[i] = ca.lk[i];
+                    lam_s_dir = ca.lam_s;
+                    v6_add(lam, ca.lam_ns);
                     have_lk = true;
                 }
             }
-            for (int i = 0; i < 5; ++i) Y[i] = sc.get6(jY + 6 * i);
             lam = dp_step_reverse(sn, ns, Y, hs, pt, g, lam, have_lk ? lk : nullptr, lam_s_dir, lam_pt, g_a);
         }
         if (seg_start == 0) break;
@@ -142,12 +175,28 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
     out[3] += g_m;
 }
 
-// Persistent adjoint grid: each wave claims 64 work items (the forward
-// pass's tile order) with one atomic; each lane reverses one ray.
+// Persistent adjoint grid: each wave claims one 8x8 pixel tile (64 work
+// items) with one atomic; each lane reverses one ray.  Neighbouring rays
+// cross the disk on the same steps, so tiles keep a wave's lanes together
+// through the crossing path; the tiles are visited in descending order of
+// their longest logged ray (grad_keys + a counting sort), longest first.
+// One wave per 8x8 tile: key = min(longest logged ray of the tile, 255).
+__global__ __launch_bounds__(256) void grad_keys(const uint32_t* __restrict__ nsteps, int W, int H, int tiles_x,
+                                                 int n_tiles, uint8_t* __restrict__ keys) {
+    const int lane = threadIdx.x & 63;
+    const int tile = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (tile >= n_tiles) return;
+    const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
+    uint32_t n = (px < W && py < H) ? nsteps[(size_t)py * W + px] : 0u;
+    for (int off = 32; off > 0; off >>= 1) n = max(n, (uint32_t)__shfl_xor((int)n, off));
+    if (lane == 0) keys[tile] = (uint8_t)min(n, 255u);
+}
+
 __global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restrict__ Ps, GradConsts G,
                                                      const float* __restrict__ dl_dhdr, double* __restrict__ grad_px,
                                                      double* __restrict__ scratch, uint32_t scratch_stride,
-                                                     unsigned long long* __restrict__ queue, uint64_t total_items) {
+                                                     unsigned long long* __restrict__ queue, uint64_t total_items,
+                                                     const uint32_t* __restrict__ order) {
     const RenderParams& P = Ps[0];
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,8 +207,10 @@ __global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restr
         base = __shfl(base, 0);
         if (base >= total_items) break;
         const uint64_t item = base + (uint64_t)lane;
-        int rid = 0, px = 0, py = 0;
-        if (item < total_items && decode_item<false>(Ps, 1, item, rid, px, py)) {
+        const uint32_t tile = item < total_items ? order[item >> 6] : 0u;
+        const int px = (int)(tile % (uint32_t)P.tiles_x) * 8 + (lane & 7);
+        const int py = (int)(tile / (uint32_t)P.tiles_x) * 8 + (lane >> 3);
+        if (item < total_items && px < P.W && py < P.H) {
             const size_t pix = (size_t)py * (size_t)P.W + (size_t)px;
             const float4 wv = *reinterpret_cast<const float4*>(dl_dhdr + 4 * pix);
             const double w[3] = {(double)wv.x, (double)wv.y, (double)wv.z};
@@ -209,12 +260,21 @@ hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t fi
     return hipGetLastError();
 }
 
+hipError_t launch_sort_desc(const uint8_t* d_keys, int n, uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream);
+
+// keys/hist/order: one byte per 8x8 tile, the sort's counters, one word
+// per tile.
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
-                                uint64_t total_items, uint32_t n_pix, double* grad_out, hipStream_t stream) {
+                                const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
+                                uint32_t n_pix, double* grad_out, hipStream_t stream) {
+    const int tiles_x = (W + 7) / 8, n_tiles = tiles_x * ((H + 7) / 8);
+    hipLaunchKernelGGL(grad_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, nsteps, W, H, tiles_x, n_tiles, keys);
+    hipError_t e = launch_sort_desc(keys, n_tiles, hist, order, stream);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(grad_backward, dim3(threads / 256), dim3(256), 0, stream, d_params, G, dl_dhdr, grad_px,
-                       scratch, threads, queue, total_items);
+                       scratch, threads, queue, (uint64_t)n_tiles * 64, order);
     hipLaunchKernelGGL(grad_reduce, dim3(1), dim3(256), 0, stream, grad_px, n_pix, grad_out);
     return hipGetLastError();
 }

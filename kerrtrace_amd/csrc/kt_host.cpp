@@ -45,7 +45,9 @@ hipError_t launch_trace_fp64_record(const RenderParams* d_params, uint64_t total
 hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t first_pool_chunk, hipStream_t stream);
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
-                                uint64_t total_items, uint32_t n_pix, double* grad_out, hipStream_t stream);
+                                const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
+                                uint32_t n_pix, double* grad_out, hipStream_t stream);
+size_t cost_order_hist_words(int n_tiles);
 size_t bloom_h_lds_bytes(int R);
 size_t post_v_lds_bytes(int R, int TH);
 }  // namespace kt
@@ -508,6 +510,9 @@ struct kt_grad {
     uint32_t* h_pool = nullptr;  // pinned copy
     double* d_grad_px = nullptr;
     double* d_scratch = nullptr;
+    uint8_t* d_rkeys = nullptr;   // adjoint order: per-pixel step-count keys,
+    uint32_t* d_rhist = nullptr;  // sort counters,
+    uint32_t* d_rorder = nullptr; // pixels by descending step count
     unsigned long long* d_queue = nullptr;
     hipEvent_t ev[4] = {};
     hipEvent_t ev_params = nullptr;
@@ -522,7 +527,7 @@ void release_grad(kt_grad* g) {
     if (!g) return;
     void* bufs[] = {g->d_disk_lut, g->d_bb_lut, g->d_dlut, g->d_params, g->d_counters, g->d_flag, g->d_cls,
                     g->d_keys, g->d_hist, g->d_perm, g->d_arena, g->d_head, g->d_nsteps, g->d_pk, g->d_pool,
-                    g->d_grad_px, g->d_scratch, g->d_queue};
+                    g->d_grad_px, g->d_scratch, g->d_queue, g->d_rkeys, g->d_rhist, g->d_rorder};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (g->h_params) (void)hipHostFree(g->h_params);
@@ -755,7 +760,8 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     KT_CREATE_HIP(hipMalloc(&r->d_trace_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
     r->max_tiles = (size_t)cfg->width * cfg->height;  // tile edge 1 is the largest count
     KT_CREATE_HIP(hipMalloc(&r->d_cost_keys, r->max_tiles), "hipMalloc cost keys");
-    KT_CREATE_HIP(hipMalloc(&r->d_cost_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_CREATE_HIP(hipMalloc(&r->d_cost_hist, kt::cost_order_hist_words((int)r->max_tiles) * sizeof(uint32_t)),
+                  "hipMalloc cost histogram");
     KT_CREATE_HIP(hipMalloc(&r->d_tile_perm, r->max_tiles * sizeof(uint32_t)), "hipMalloc tile order");
     KT_CREATE_HIP(hipMemset(r->d_trace_flag, 0, 2 * sizeof(uint32_t)), "clear flag");
     KT_CREATE_HIP(hipHostMalloc(&r->h_params, sizeof(kt::RenderParams) * kt_renderer::kRing, hipHostMallocDefault), "pinned params");
@@ -1006,7 +1012,8 @@ kt_status kt_batch_create(const kt_config* cfgs, const kt_camera* cams, int32_t 
     for (int i = 0; i < n; ++i) tiles += (size_t)((cfgs[i].width + 1) / 2) * ((cfgs[i].height + 1) / 2);
     b->max_tiles = tiles;
     KT_BATCH_HIP(hipMalloc(&b->d_keys, tiles), "hipMalloc cost keys");
-    KT_BATCH_HIP(hipMalloc(&b->d_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_BATCH_HIP(hipMalloc(&b->d_hist, kt::cost_order_hist_words((int)tiles) * sizeof(uint32_t)),
+                 "hipMalloc cost histogram");
     KT_BATCH_HIP(hipMalloc(&b->d_perm, tiles * sizeof(uint32_t)), "hipMalloc tile order");
     KT_BATCH_HIP(hipEventCreate(&b->ev0), "event");
     KT_BATCH_HIP(hipEventCreate(&b->ev1), "event");
@@ -1177,7 +1184,8 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
     KT_GRAD_HIP(hipMalloc(&g->d_cls, n_pix), "hipMalloc classes");
     const size_t tiles = (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8);
     KT_GRAD_HIP(hipMalloc(&g->d_keys, tiles), "hipMalloc cost keys");
-    KT_GRAD_HIP(hipMalloc(&g->d_hist, 512 * sizeof(uint32_t)), "hipMalloc cost histogram");
+    KT_GRAD_HIP(hipMalloc(&g->d_hist, kt::cost_order_hist_words((int)tiles) * sizeof(uint32_t)),
+                "hipMalloc cost histogram");
     KT_GRAD_HIP(hipMalloc(&g->d_perm, tiles * sizeof(uint32_t)), "hipMalloc tile order");
     KT_GRAD_HIP(hipMalloc(&g->d_arena, (size_t)cap * g->stride * sizeof(double)), "hipMalloc step log");
     KT_GRAD_HIP(hipMalloc(&g->d_head, n_pix * sizeof(uint32_t)), "hipMalloc log heads");
@@ -1186,9 +1194,13 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
     KT_GRAD_HIP(hipMalloc(&g->d_pool, 2 * sizeof(uint32_t)), "hipMalloc pool head");
     KT_GRAD_HIP(hipMalloc(&g->d_grad_px, 4 * n_pix * sizeof(double)), "hipMalloc per-pixel gradient");
     g->threads = (uint32_t)std::max(256, (g->cus * 8 + 3) / 4 * 256);
-    const size_t sc = (size_t)(6 * g->K + 30) * g->threads;
+    const size_t sc = (size_t)(36 * g->K + 6) * g->threads;
     KT_GRAD_HIP(hipMalloc(&g->d_scratch, sc * sizeof(double)), "hipMalloc adjoint scratch");
     KT_GRAD_HIP(hipMalloc(&g->d_queue, sizeof(unsigned long long)), "hipMalloc queue");
+    KT_GRAD_HIP(hipMalloc(&g->d_rkeys, n_pix), "hipMalloc adjoint keys");
+    KT_GRAD_HIP(hipMalloc(&g->d_rhist, kt::cost_order_hist_words((int)n_pix) * sizeof(uint32_t)),
+                "hipMalloc adjoint sort counters");
+    KT_GRAD_HIP(hipMalloc(&g->d_rorder, n_pix * sizeof(uint32_t)), "hipMalloc adjoint order");
     for (auto& e : g->ev) KT_GRAD_HIP(hipEventCreate(&e), "event");
     KT_GRAD_HIP(hipEventCreateWithFlags(&g->ev_params, hipEventDisableTiming), "event");
 #undef KT_GRAD_HIP
@@ -1296,7 +1308,8 @@ kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad, doubl
     e = hipEventRecord(g->ev[2], s);
     if (e == hipSuccess)
         e = kt::launch_grad_backward(g->d_params, G, dl_dhdr, grad_px ? grad_px : g->d_grad_px, g->d_scratch,
-                                     g->threads, g->d_queue, g->items, g->n_pix, grad, s);
+                                     g->threads, g->d_queue, g->d_nsteps, g->d_rkeys, g->d_rhist, g->d_rorder,
+                                     (int)g->cfg.width, (int)g->cfg.height, g->n_pix, grad, s);
     if (e == hipSuccess) e = hipEventRecord(g->ev[3], s);
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("adjoint: ") + hipGetErrorString(e));
     g->last_stream = s;
@@ -1324,6 +1337,86 @@ kt_status kt_grad_get_stats(kt_grad* g, kt_grad_stats* out) {
     out->chunks = pool[0];
     out->log_overflow = pool[1];
     out->checkpoint_interval = g->K;
+    return KT_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+// Candidate launch options: resident waves per CU (one or two per SIMD),
+// idle lanes that trigger a refill (how many rays a wave carries before it
+// takes new work) and the tile edge of the work order.
+static const kt_launch_opts kTuneGrid[] = {
+    {8, 8, 8, 1, 256},  {8, 16, 8, 1, 256}, {8, 32, 8, 1, 256}, {8, 8, 4, 1, 256},
+    {8, 16, 4, 1, 256}, {4, 8, 8, 1, 256},  {4, 16, 8, 1, 256}, {4, 32, 8, 1, 256},
+    {4, 16, 4, 1, 256}, {4, 8, 4, 1, 256},
+};
+static const int kTuneReps = 2;
+
+kt_status kt_autotune(kt_renderer* r, const kt_camera* cam, float* hdr, void* stream, kt_launch_opts* best) {
+    if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    KT_HIP(r, hipEventCreate(&e0), "event");
+    KT_HIP(r, hipEventCreate(&e1), "event");
+    float best_ms = 3.0e38f;
+    kt_launch_opts best_o = r->opts;
+    int rc = KT_OK;
+    for (const kt_launch_opts& o : kTuneGrid) {
+        r->opts = o;
+        float t_min = 3.0e38f;
+        for (int k = 0; k < kTuneReps && rc == KT_OK; ++k) {
+            hipError_t e = hipEventRecord(e0, s);
+            if (e == hipSuccess) rc = trace_into(r, cam, hdr, nullptr, r->d_trace_flag, s);
+            if (rc == KT_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+            if (rc == KT_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (rc == KT_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess && rc == KT_OK) rc = hip_err(r, e, "autotune timing");
+            if (rc == KT_OK) t_min = std::min(t_min, ms);
+        }
+        if (rc != KT_OK) break;
+        if (t_min < best_ms) {
+            best_ms = t_min;
+            best_o = o;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    r->opts = best_o;
+    if (best) *best = best_o;
+    return rc;
+}
+
+kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream, kt_launch_opts* best) {
+    if (!b || !hdr) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    float best_ms = 3.0e38f;
+    kt_launch_opts best_o = b->opts;
+    for (const kt_launch_opts& o : kTuneGrid) {
+        b->opts = o;
+        float t_min = 3.0e38f;
+        for (int k = 0; k < kTuneReps; ++k) {
+            int rc = kt_batch_trace(b, hdr, nullptr, stream);
+            if (rc != KT_OK) {
+                b->opts = best_o;
+                return rc;
+            }
+            hipError_t e = hipEventSynchronize(b->ev1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+            if (e != hipSuccess) {
+                b->opts = best_o;
+                return hip_err(nullptr, e, "autotune timing");
+            }
+            t_min = std::min(t_min, ms);
+        }
+        if (t_min < best_ms) {
+            best_ms = t_min;
+            best_o = o;
+        }
+    }
+    (void)s;
+    b->opts = best_o;
+    if (best) *best = best_o;
     return KT_OK;
 }
 }  // extern "C"

@@ -101,58 +101,85 @@ __global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict_
     }
 }
 
-__global__ void key_histogram(const uint8_t* __restrict__ keys, int n, uint32_t* __restrict__ hist) {
+// Counting sort of the tile keys in descending order as a block-local
+// multi-split: each block owns a contiguous chunk of tiles, counts its keys
+// in LDS (key_histogram), one block turns the (bucket, block) counts into
+// start offsets, bucket-major (key_scan), and each block places its tiles
+// at its offsets plus an LDS rank (key_scatter).  No global atomics.
+static constexpr int kSortChunk = 4096;  // tiles per block
+
+__global__ __launch_bounds__(256) void key_histogram(const uint8_t* __restrict__ keys, int n,
+                                                     uint32_t* __restrict__ block_hist) {
     __shared__ uint32_t h[kBuckets];
     for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        atomicAdd(&h[kBuckets - 1 - keys[i]], 1u);  // descending key order
+    const int begin = blockIdx.x * kSortChunk, end = min(n, begin + kSortChunk);
+    for (int i = begin + (int)threadIdx.x; i < end; i += blockDim.x) atomicAdd(&h[kBuckets - 1 - keys[i]], 1u);
     __syncthreads();
-    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
+    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) block_hist[(size_t)blockIdx.x * kBuckets + i] = h[i];
 }
 
-// Exclusive scan of the 256 bucket counts (one block of 256 threads).
-__global__ void key_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t s[kBuckets];
-    const int t = threadIdx.x;
-    s[t] = hist[t];
+// One block of 256 threads: thread b walks bucket b over all blocks, then
+// an exclusive scan over the bucket totals places the buckets.
+__global__ __launch_bounds__(256) void key_scan(uint32_t* __restrict__ block_hist, int n_blocks) {
+    __shared__ uint32_t tot[kBuckets];
+    const int b = threadIdx.x;
+    uint32_t sum = 0;
+    for (int k = 0; k < n_blocks; ++k) {
+        uint32_t c = block_hist[(size_t)k * kBuckets + b];
+        block_hist[(size_t)k * kBuckets + b] = sum;  // offset within the bucket
+        sum += c;
+    }
+    tot[b] = sum;
     __syncthreads();
     for (int d = 1; d < kBuckets; d <<= 1) {
-        uint32_t v = t >= d ? s[t - d] : 0u;
+        uint32_t v = b >= d ? tot[b - d] : 0u;
         __syncthreads();
-        s[t] += v;
+        tot[b] += v;
         __syncthreads();
     }
-    offsets[t] = s[t] - hist[t];
+    const uint32_t bucket_start = tot[b] - sum;
+    for (int k = 0; k < n_blocks; ++k) block_hist[(size_t)k * kBuckets + b] += bucket_start;
 }
 
-__global__ void key_scatter(const uint8_t* __restrict__ keys, int n, uint32_t* __restrict__ offsets,
-                            uint32_t* __restrict__ perm) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t pos = atomicAdd(&offsets[kBuckets - 1 - keys[i]], 1u);
+__global__ __launch_bounds__(256) void key_scatter(const uint8_t* __restrict__ keys, int n,
+                                                   const uint32_t* __restrict__ block_off,
+                                                   uint32_t* __restrict__ perm) {
+    __shared__ uint32_t off[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) off[i] = block_off[(size_t)blockIdx.x * kBuckets + i];
+    __syncthreads();
+    const int begin = blockIdx.x * kSortChunk, end = min(n, begin + kSortChunk);
+    for (int i = begin + (int)threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t pos = atomicAdd(&off[kBuckets - 1 - keys[i]], 1u);
         perm[pos] = (uint32_t)i;
     }
 }
 
-__global__ void zero_u32(uint32_t* p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 // Builds perm (n_tiles entries): tile indices in descending predicted cost.
-// scratch: keys (n_tiles bytes, 4-byte aligned region) + 2 * 256 uint32.
+// d_hist holds kBuckets counters per sort block (ceil(n_tiles / 4096)).
 hipError_t launch_cost_order(const RenderParams* d_params, int n_renders, int n_tiles, uint8_t* d_keys,
                              uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream) {
-    uint32_t* d_off = d_hist + kBuckets;
-    hipLaunchKernelGGL(zero_u32, dim3(2), dim3(256), 0, stream, d_hist, 2 * kBuckets);
+    const int n_blocks = (n_tiles + kSortChunk - 1) / kSortChunk;
     hipLaunchKernelGGL(cost_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, d_params, n_renders, d_keys,
                        n_tiles);
-    int blocks = min(1024, (n_tiles + 255) / 256);
-    hipLaunchKernelGGL(key_histogram, dim3(blocks), dim3(256), 0, stream, d_keys, n_tiles, d_hist);
-    hipLaunchKernelGGL(key_scan, dim3(1), dim3(kBuckets), 0, stream, d_hist, d_off);
-    hipLaunchKernelGGL(key_scatter, dim3(blocks), dim3(256), 0, stream, d_keys, n_tiles, d_off, d_perm);
+    hipLaunchKernelGGL(key_histogram, dim3(n_blocks), dim3(256), 0, stream, d_keys, n_tiles, d_hist);
+    hipLaunchKernelGGL(key_scan, dim3(1), dim3(kBuckets), 0, stream, d_hist, n_blocks);
+    hipLaunchKernelGGL(key_scatter, dim3(n_blocks), dim3(256), 0, stream, d_keys, n_tiles, d_hist, d_perm);
     return hipGetLastError();
+}
+
+// Order of n items by descending 8-bit key (the same multi-split).
+hipError_t launch_sort_desc(const uint8_t* d_keys, int n, uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream) {
+    const int n_blocks = (n + kSortChunk - 1) / kSortChunk;
+    hipLaunchKernelGGL(key_histogram, dim3(n_blocks), dim3(256), 0, stream, d_keys, n, d_hist);
+    hipLaunchKernelGGL(key_scan, dim3(1), dim3(kBuckets), 0, stream, d_hist, n_blocks);
+    hipLaunchKernelGGL(key_scatter, dim3(n_blocks), dim3(256), 0, stream, d_keys, n, d_hist, d_perm);
+    return hipGetLastError();
+}
+
+// Counter words the tile order needs for n_tiles tiles.
+size_t cost_order_hist_words(int n_tiles) {
+    return (size_t)((n_tiles + kSortChunk - 1) / kSortChunk) * kBuckets;
 }
 
 }  // namespace kt

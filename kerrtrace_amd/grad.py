@@ -164,3 +164,22 @@ def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.0
             print(f"[adam] it {it} loss {history[-1][0]:.6e} theta {history[-1][1]}", flush=True)
     theta = (z * scale).detach()
     return [float(v) for v in theta], history
+
+
+def autotune_checkpoint(cfg, theta, intervals=(8, 16, 32, 64), fov_deg=GRAD_FOV_DEG, log_steps_per_ray=512, reps=2):
+    """Checkpoint interval whose adjoint pass is fastest for this scene:
+    for each interval a renderer traces theta and runs the backward pass
+    (unit loss weights) `reps` times.  Returns (best interval, {interval:
+    best backward ms}).  Gradients do not depend on the interval."""
+    torch = _torch()
+    times = {}
+    for k in intervals:
+        with GradRenderer(cfg, fov_deg=fov_deg, checkpoint_interval=k, log_steps_per_ray=log_steps_per_ray) as g:
+            hdr, _ = g.forward(theta)
+            w = torch.ones_like(hdr)
+            best = float("inf")
+            for _ in range(reps):
+                g.backward(w)
+                best = min(best, g.stats()["backward_ms"])
+            times[k] = best
+    return min(times, key=times.get), times

@@ -75,6 +75,18 @@ class HipRenderer:
         o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
         check(lib().kt_set_launch_opts(self._h, C.byref(o)), self._h)
 
+    def autotune(self, cam, hdr=None, stream=None):
+        """Pick the fastest launch options for this frame (kt_autotune);
+        returns them as a dict.  hdr is overwritten."""
+        torch = _torch()
+        if hdr is None:
+            hdr = self.alloc_hdr()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        o = KtLaunchOpts()
+        check(lib().kt_autotune(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()), C.c_void_p(s.cuda_stream),
+                                C.byref(o)), self._h)
+        return {name: getattr(o, name) for name, _ in o._fields_}
+
     def alloc_hdr(self):
         torch = _torch()
         return torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
@@ -201,6 +213,19 @@ class HipBatch:
     def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0):
         o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
         check(lib().kt_batch_set_launch_opts(self._h, C.byref(o)))
+
+    def autotune(self, hdr=None, stream=None):
+        """Pick the fastest launch options for the batch (kt_batch_autotune);
+        returns them as a dict.  The hdr tensors are overwritten."""
+        torch = _torch()
+        if hdr is None:
+            hdr, _ = self.alloc(with_class=False)
+        n = len(self.cfgs)
+        hp = (C.c_void_p * n)(*[t.data_ptr() for t in hdr])
+        s = stream if stream is not None else torch.cuda.current_stream()
+        o = KtLaunchOpts()
+        check(lib().kt_batch_autotune(self._h, hp, C.c_void_p(s.cuda_stream), C.byref(o)))
+        return {name: getattr(o, name) for name, _ in o._fields_}
 
     def alloc(self, with_class=True):
         torch = _torch()

@@ -99,3 +99,27 @@ def test_batch_matches_oracle(torch_gpu):
             say(f"[batch] oracle render {k}: {m}")
             assert m["class_mismatch"] == 0
             assert m["exact_fp32_frac"] == 1.0
+
+
+def test_autotune_keeps_frames_identical(torch_gpu):
+    """The autotuner only picks launch options; the frame it leaves behind
+    equals the default-options frame bit for bit."""
+    from kerrtrace_amd.renderer import HipBatch, HipRenderer
+    cfg, cam = K.scaled(*K.preset("hd1080_a0998"), 192, 108)
+    with HipRenderer(cfg) as r:
+        ref, _ = r.trace(cam)
+        ref = ref.cpu().numpy()
+        opts = r.autotune(cam)
+        say(f"[autotune] single: {opts}")
+        assert opts["waves_per_cu"] in (4, 8) and opts["refill_threshold"] in (8, 16, 32)
+        hdr, _ = r.trace(cam)
+        assert np.array_equal(hdr.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    renders = _renders(4, 48)
+    with HipBatch(renders) as b:
+        ref, _ = b.trace()
+        ref = [t.cpu().numpy() for t in ref]
+        opts = b.autotune()
+        say(f"[autotune] batch: {opts}")
+        hdr, _ = b.trace()
+        for k in range(len(renders)):
+            assert np.array_equal(hdr[k].cpu().numpy().view(np.uint32), ref[k].view(np.uint32))
