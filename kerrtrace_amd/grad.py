@@ -132,14 +132,16 @@ def render(theta, renderer):
     return _KERR_RENDER.apply(theta, renderer)
 
 
-def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), final_lr=0.05, log_every=0):
+def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), final_lr=1.0, log_every=0):
     """Fit theta to a target frame with Adam on the L2 loss over the linear
     HDR colour channels.  Adam's step is about its learning rate in each
     coordinate whatever the gradient's scale, so theta is optimised as
     theta = step * z on z: `step` is the initial per-iteration step size in
-    spin, inclination [rad], r_obs [M] and mdot, and the learning rate decays
-    geometrically from 1 to `final_lr` over the run.  Returns (theta [4]
-    list, history of (loss, theta) before each update)."""
+    spin, inclination [rad], r_obs [M] and mdot; the learning rate decays
+    geometrically from 1 to `final_lr` over the run (1: constant, which
+    suits this loss: its frozen-event gradient ignores the image's
+    discontinuities, and a decaying step stalls in the ripples they leave).
+    Returns (theta [4] list, history of (loss, theta) before each update)."""
     torch = _torch()
     scale = torch.tensor(step, dtype=torch.float64)
     z = (torch.tensor([float(v) for v in theta0], dtype=torch.float64) / scale).requires_grad_(True)
