@@ -225,23 +225,37 @@ __global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restr
     }
 }
 
-// Deterministic sum of the per-pixel contributions: one block of 256
-// threads, each summing a fixed strided subset, then a fixed tree.
-__global__ __launch_bounds__(256) void grad_reduce(const double* __restrict__ grad_px, uint32_t n_pix,
-                                                   double* __restrict__ out) {
-    __shared__ double part[4][256];
+// Deterministic sum of the per-pixel contributions in two passes: block b
+// of kReduceBlocks sums a fixed strided subset of pixels with a fixed tree
+// into part[b]; one block then sums the parts the same way.
+static constexpr int kReduceBlocks = 256;
+__global__ __launch_bounds__(256) void grad_reduce_part(const double* __restrict__ grad_px, uint32_t n_pix,
+                                                        double* __restrict__ part) {
+    __shared__ double sh[4][256];
     const int t = threadIdx.x;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = (uint32_t)t; i < n_pix; i += 256)
+    for (uint32_t i = blockIdx.x * 256u + (uint32_t)t; i < n_pix; i += kReduceBlocks * 256u)
         for (int k = 0; k < 4; ++k) acc[k] += grad_px[4 * (size_t)i + k];
-    for (int k = 0; k < 4; ++k) part[k][t] = acc[k];
+    for (int k = 0; k < 4; ++k) sh[k][t] = acc[k];
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if (t < off)
-            for (int k = 0; k < 4; ++k) part[k][t] += part[k][t + off];
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + off];
         __syncthreads();
     }
-    if (t < 4) out[t] = part[t][0];
+    if (t < 4) part[4 * blockIdx.x + t] = sh[t][0];
+}
+__global__ __launch_bounds__(256) void grad_reduce_final(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[4][256];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 4; ++k) sh[k][t] = part[4 * t + k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + off];
+        __syncthreads();
+    }
+    if (t < 4) out[t] = sh[t][0];
 }
 
 __global__ void zero_u64(unsigned long long* p) {
@@ -263,11 +277,11 @@ hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t fi
 hipError_t launch_sort_desc(const uint8_t* d_keys, int n, uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream);
 
 // keys/hist/order: one byte per 8x8 tile, the sort's counters, one word
-// per tile.
+// per tile; part: 4 * 256 doubles.
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
                                 const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
-                                uint32_t n_pix, double* grad_out, hipStream_t stream) {
+                                uint32_t n_pix, double* part, double* grad_out, hipStream_t stream) {
     const int tiles_x = (W + 7) / 8, n_tiles = tiles_x * ((H + 7) / 8);
     hipLaunchKernelGGL(grad_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, nsteps, W, H, tiles_x, n_tiles, keys);
     hipError_t e = launch_sort_desc(keys, n_tiles, hist, order, stream);
@@ -275,7 +289,8 @@ hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& 
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(grad_backward, dim3(threads / 256), dim3(256), 0, stream, d_params, G, dl_dhdr, grad_px,
                        scratch, threads, queue, (uint64_t)n_tiles * 64, order);
-    hipLaunchKernelGGL(grad_reduce, dim3(1), dim3(256), 0, stream, grad_px, n_pix, grad_out);
+    hipLaunchKernelGGL(grad_reduce_part, dim3(kReduceBlocks), dim3(256), 0, stream, grad_px, n_pix, part);
+    hipLaunchKernelGGL(grad_reduce_final, dim3(1), dim3(256), 0, stream, part, grad_out);
     return hipGetLastError();
 }
 

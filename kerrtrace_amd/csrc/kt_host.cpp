@@ -46,7 +46,7 @@ hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t fi
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
                                 const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
-                                uint32_t n_pix, double* grad_out, hipStream_t stream);
+                                uint32_t n_pix, double* part, double* grad_out, hipStream_t stream);
 size_t cost_order_hist_words(int n_tiles);
 size_t bloom_h_lds_bytes(int R);
 size_t post_v_lds_bytes(int R, int TH);
@@ -512,7 +512,8 @@ struct kt_grad {
     double* d_scratch = nullptr;
     uint8_t* d_rkeys = nullptr;   // adjoint order: per-pixel step-count keys,
     uint32_t* d_rhist = nullptr;  // sort counters,
-    uint32_t* d_rorder = nullptr; // pixels by descending step count
+    uint32_t* d_rorder = nullptr; // tiles by descending longest ray
+    double* d_part = nullptr;     // partial gradient sums
     unsigned long long* d_queue = nullptr;
     hipEvent_t ev[4] = {};
     hipEvent_t ev_params = nullptr;
@@ -527,7 +528,7 @@ void release_grad(kt_grad* g) {
     if (!g) return;
     void* bufs[] = {g->d_disk_lut, g->d_bb_lut, g->d_dlut, g->d_params, g->d_counters, g->d_flag, g->d_cls,
                     g->d_keys, g->d_hist, g->d_perm, g->d_arena, g->d_head, g->d_nsteps, g->d_pk, g->d_pool,
-                    g->d_grad_px, g->d_scratch, g->d_queue, g->d_rkeys, g->d_rhist, g->d_rorder};
+                    g->d_grad_px, g->d_scratch, g->d_queue, g->d_rkeys, g->d_rhist, g->d_rorder, g->d_part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (g->h_params) (void)hipHostFree(g->h_params);
@@ -1201,6 +1202,7 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
     KT_GRAD_HIP(hipMalloc(&g->d_rhist, kt::cost_order_hist_words((int)n_pix) * sizeof(uint32_t)),
                 "hipMalloc adjoint sort counters");
     KT_GRAD_HIP(hipMalloc(&g->d_rorder, n_pix * sizeof(uint32_t)), "hipMalloc adjoint order");
+    KT_GRAD_HIP(hipMalloc(&g->d_part, 4 * 256 * sizeof(double)), "hipMalloc partial sums");
     for (auto& e : g->ev) KT_GRAD_HIP(hipEventCreate(&e), "event");
     KT_GRAD_HIP(hipEventCreateWithFlags(&g->ev_params, hipEventDisableTiming), "event");
 #undef KT_GRAD_HIP
@@ -1309,7 +1311,7 @@ kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad, doubl
     if (e == hipSuccess)
         e = kt::launch_grad_backward(g->d_params, G, dl_dhdr, grad_px ? grad_px : g->d_grad_px, g->d_scratch,
                                      g->threads, g->d_queue, g->d_nsteps, g->d_rkeys, g->d_rhist, g->d_rorder,
-                                     (int)g->cfg.width, (int)g->cfg.height, g->n_pix, grad, s);
+                                     (int)g->cfg.width, (int)g->cfg.height, g->n_pix, g->d_part, grad, s);
     if (e == hipSuccess) e = hipEventRecord(g->ev[3], s);
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("adjoint: ") + hipGetErrorString(e));
     g->last_stream = s;
