@@ -236,7 +236,10 @@ kt_status kt_ready_frame(kt_renderer* r, kt_frame_view* view,
 kt_status kt_finish_frame(kt_renderer* r, int32_t index);
 
 /* ---- stream-ordered device API on caller-owned buffers --------------- */
-/* hdr: float4[W*H] (rgb, 1); hit_class: uint8[W*H] or NULL. */
+/* hdr: float4[W*H] (rgb, 1); hit_class: uint8[W*H] or NULL.
+ * Consecutive traces use two alternating sets of launch state (work queue,
+ * counters, tile order), so traces issued on two different streams run
+ * concurrently on the device; a third waits for the first to finish. */
 kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr,
                    uint8_t* hit_class, void* stream);
 /* As kt_trace, also writing each pixel's DP5(4) attempt count to
@@ -245,7 +248,7 @@ kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr,
 kt_status kt_trace_ex(kt_renderer* r, const kt_camera* cam, float* hdr,
                       uint8_t* hit_class, uint32_t* attempts, void* stream);
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* opts);
-/* Synchronises the renderer stream and returns the last trace's counters. */
+/* Waits for the last issued trace and returns its counters. */
 kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out);
 /* bloom: float4[W*H] scratch; ldr: uint32[W*H] 0x00RRGGBB. */
 kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom,

@@ -206,19 +206,30 @@ struct kt_renderer {
     int device = 0;
     int cus = 256;
     size_t max_lds = 65536;
-    hipStream_t stream = nullptr;
     double* d_disk_lut = nullptr;
     double* d_bb_lut = nullptr;
     float* d_disk_lut_f = nullptr;
     float* d_bb_lut_f = nullptr;
     float* d_bloom_w = nullptr;
     int post_th = 64;
-    kt::DevCounters* d_counters = nullptr;
-    uint32_t* d_trace_flag = nullptr;
-    // hardest-first tile order: keys, bucket counts/offsets, permutation
-    uint8_t* d_cost_keys = nullptr;
-    uint32_t* d_cost_hist = nullptr;
-    uint32_t* d_tile_perm = nullptr;
+    // Per-launch state in two slots that traces use in turn, so that two
+    // traces issued on two streams run concurrently: the long-ray tail of
+    // one frame then overlaps the bulk of the next.  A slot is reused only
+    // after its previous trace has finished (stream wait on `done`).
+    struct Slot {
+        kt::DevCounters* counters = nullptr;
+        uint32_t* flag = nullptr;       // LUT-overflow flag of kt_trace launches
+        uint32_t* flag_used = nullptr;  // the flag the slot's last trace wrote
+        // hardest-first tile order: keys, bucket counts/offsets, permutation
+        uint8_t* keys = nullptr;
+        uint32_t* hist = nullptr;
+        uint32_t* perm = nullptr;
+        hipEvent_t ev0 = nullptr, done = nullptr;  // around the tracer launch
+        bool used = false;
+        uint64_t rays = 0;
+    };
+    Slot slots[2];
+    int next_slot = 0, last_slot = -1;
     size_t max_tiles = 0;
     // ring of parameter records: pinned host staging -> device
     static constexpr int kRing = 16;
@@ -228,10 +239,7 @@ struct kt_renderer {
     bool param_ev_used[kRing] = {};
     int param_next = 0;
     kt_launch_opts opts{8, 8, 8, 1, 256};
-    hipStream_t last_stream = nullptr;
-    uint64_t last_rays = 0;
-    hipEvent_t ev_trace0 = nullptr, ev_trace1 = nullptr;  // around the tracer launch
-    // double-buffered frames (mod.rs:24-51, 88-91)
+    // double-buffered frames (mod.rs:24-51, 88-91), each on its own stream
     struct Frame {
         float* hdr = nullptr;
         float* bloom = nullptr;
@@ -240,6 +248,7 @@ struct kt_renderer {
         uint32_t* flag = nullptr;
         uint32_t* flag_host = nullptr;
         hipEvent_t ready = nullptr;
+        hipStream_t stream = nullptr;
         bool pending = false;
     };
     Frame frames[2];
@@ -276,11 +285,19 @@ void release(kt_renderer* r) {
         if (f.ldr_host) (void)hipHostFree(f.ldr_host);
         if (f.flag_host) (void)hipHostFree(f.flag_host);
         if (f.ready) (void)hipEventDestroy(f.ready);
+        if (f.stream) (void)hipStreamDestroy(f.stream);
+    }
+    for (auto& sl : r->slots) {
+        if (sl.counters) (void)hipFree(sl.counters);
+        if (sl.flag) (void)hipFree(sl.flag);
+        if (sl.keys) (void)hipFree(sl.keys);
+        if (sl.hist) (void)hipFree(sl.hist);
+        if (sl.perm) (void)hipFree(sl.perm);
+        if (sl.ev0) (void)hipEventDestroy(sl.ev0);
+        if (sl.done) (void)hipEventDestroy(sl.done);
     }
     for (int i = 0; i < kt_renderer::kRing; ++i)
         if (r->param_ev[i]) (void)hipEventDestroy(r->param_ev[i]);
-    if (r->ev_trace0) (void)hipEventDestroy(r->ev_trace0);
-    if (r->ev_trace1) (void)hipEventDestroy(r->ev_trace1);
     if (r->h_params) (void)hipHostFree(r->h_params);
     if (r->d_params) (void)hipFree(r->d_params);
     if (r->d_disk_lut) (void)hipFree(r->d_disk_lut);
@@ -288,12 +305,6 @@ void release(kt_renderer* r) {
     if (r->d_disk_lut_f) (void)hipFree(r->d_disk_lut_f);
     if (r->d_bb_lut_f) (void)hipFree(r->d_bb_lut_f);
     if (r->d_bloom_w) (void)hipFree(r->d_bloom_w);
-    if (r->d_counters) (void)hipFree(r->d_counters);
-    if (r->d_trace_flag) (void)hipFree(r->d_trace_flag);
-    if (r->d_cost_keys) (void)hipFree(r->d_cost_keys);
-    if (r->d_cost_hist) (void)hipFree(r->d_cost_hist);
-    if (r->d_tile_perm) (void)hipFree(r->d_tile_perm);
-    if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
 }
 
@@ -373,8 +384,13 @@ int upload_params(kt_renderer* r, const kt::RenderParams& P, hipStream_t s, kt::
     return KT_OK;
 }
 
+// Traces one frame on stream s with the next per-launch slot; flag null
+// means the slot's own LUT-overflow flag.
 int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* flag, hipStream_t s,
                uint32_t* attempts = nullptr) {
+    kt_renderer::Slot& sl = r->slots[r->next_slot];
+    if (!flag) flag = sl.flag;
+    if (sl.used) KT_HIP(r, hipStreamWaitEvent(s, sl.done, 0), "slot wait");
     kt::RenderParams P;
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
     make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
@@ -382,26 +398,29 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     const bool order = r->opts.cost_order != 2 && tile <= 8;
     const int n_tiles = (int)P.n_tiles;
     const uint64_t items = (uint64_t)n_tiles * (uint64_t)tile * (uint64_t)tile;
-    if (order) P.tile_perm = r->d_tile_perm;
+    if (order) P.tile_perm = sl.perm;
     kt::RenderParams* dp = nullptr;
     int rc = upload_params(r, P, s, &dp);
     if (rc != KT_OK) return rc;
     if (order) {
-        hipError_t e = kt::launch_cost_order(dp, 1, n_tiles, r->d_cost_keys, r->d_cost_hist, r->d_tile_perm, s);
+        hipError_t e = kt::launch_cost_order(dp, 1, n_tiles, sl.keys, sl.hist, sl.perm, s);
         if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("cost order launch: ") + hipGetErrorString(e));
     }
-    KT_HIP(r, kt::launch_reset(r->d_counters, flag, s), "counter reset launch");
+    KT_HIP(r, kt::launch_reset(sl.counters, flag, s), "counter reset launch");
     int wpc = r->opts.waves_per_cu > 0 ? r->opts.waves_per_cu : 8;
     int blocks = std::max(1, (r->cus * wpc + 3) / 4);
     int thr = r->opts.refill_threshold > 0 ? std::min(r->opts.refill_threshold, 64) : 8;
-    KT_HIP(r, hipEventRecord(r->ev_trace0, s), "trace event");
+    KT_HIP(r, hipEventRecord(sl.ev0, s), "trace event");
     hipError_t e = r->cfg.precision == KT_PRECISION_MIXED
-                       ? kt::launch_trace_mixed(dp, 1, items, thr, r->d_counters, blocks, s)
-                       : kt::launch_trace_fp64(dp, 1, items, thr, r->d_counters, blocks, s);
+                       ? kt::launch_trace_mixed(dp, 1, items, thr, sl.counters, blocks, s)
+                       : kt::launch_trace_fp64(dp, 1, items, thr, sl.counters, blocks, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
-    KT_HIP(r, hipEventRecord(r->ev_trace1, s), "trace event");
-    r->last_stream = s;
-    r->last_rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
+    KT_HIP(r, hipEventRecord(sl.done, s), "trace event");
+    sl.used = true;
+    sl.flag_used = flag;
+    sl.rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
+    r->last_slot = r->next_slot;
+    r->next_slot ^= 1;
     return KT_OK;
 }
 
@@ -721,7 +740,6 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     KT_CREATE_HIP(hipGetDeviceProperties(&prop, r->device), "hipGetDeviceProperties");
     r->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     r->max_lds = prop.sharedMemPerBlock > 0 ? prop.sharedMemPerBlock : 65536;
-    KT_CREATE_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking), "hipStreamCreate");
     const int n = (int)cfg->lut_size;
     std::vector<float> disk_f(n), bb_f((size_t)n * 3);
     for (int i = 0; i < n; ++i) disk_f[i] = (float)r->d.disk_lut[i];
@@ -756,21 +774,23 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     }
     KT_CREATE_HIP(hipMalloc(&r->d_bloom_w, sizeof(float) * w.size()), "hipMalloc bloom weights");
     KT_CREATE_HIP(hipMemcpy(r->d_bloom_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice), "upload bloom weights");
-    KT_CREATE_HIP(hipMalloc(&r->d_counters, sizeof(kt::DevCounters)), "hipMalloc counters");
-    KT_CREATE_HIP(hipMemset(r->d_counters, 0, sizeof(kt::DevCounters)), "clear counters");
-    KT_CREATE_HIP(hipMalloc(&r->d_trace_flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
     r->max_tiles = (size_t)cfg->width * cfg->height;  // tile edge 1 is the largest count
-    KT_CREATE_HIP(hipMalloc(&r->d_cost_keys, r->max_tiles), "hipMalloc cost keys");
-    KT_CREATE_HIP(hipMalloc(&r->d_cost_hist, kt::cost_order_hist_words((int)r->max_tiles) * sizeof(uint32_t)),
-                  "hipMalloc cost histogram");
-    KT_CREATE_HIP(hipMalloc(&r->d_tile_perm, r->max_tiles * sizeof(uint32_t)), "hipMalloc tile order");
-    KT_CREATE_HIP(hipMemset(r->d_trace_flag, 0, 2 * sizeof(uint32_t)), "clear flag");
+    for (auto& sl : r->slots) {
+        KT_CREATE_HIP(hipMalloc(&sl.counters, sizeof(kt::DevCounters)), "hipMalloc counters");
+        KT_CREATE_HIP(hipMemset(sl.counters, 0, sizeof(kt::DevCounters)), "clear counters");
+        KT_CREATE_HIP(hipMalloc(&sl.flag, 2 * sizeof(uint32_t)), "hipMalloc flag");
+        KT_CREATE_HIP(hipMemset(sl.flag, 0, 2 * sizeof(uint32_t)), "clear flag");
+        KT_CREATE_HIP(hipMalloc(&sl.keys, r->max_tiles), "hipMalloc cost keys");
+        KT_CREATE_HIP(hipMalloc(&sl.hist, kt::cost_order_hist_words((int)r->max_tiles) * sizeof(uint32_t)),
+                      "hipMalloc cost histogram");
+        KT_CREATE_HIP(hipMalloc(&sl.perm, r->max_tiles * sizeof(uint32_t)), "hipMalloc tile order");
+        KT_CREATE_HIP(hipEventCreate(&sl.ev0), "trace event");
+        KT_CREATE_HIP(hipEventCreate(&sl.done), "trace event");
+    }
     KT_CREATE_HIP(hipHostMalloc(&r->h_params, sizeof(kt::RenderParams) * kt_renderer::kRing, hipHostMallocDefault), "pinned params");
     KT_CREATE_HIP(hipMalloc(&r->d_params, sizeof(kt::RenderParams) * kt_renderer::kRing), "hipMalloc params");
     for (int i = 0; i < kt_renderer::kRing; ++i)
         KT_CREATE_HIP(hipEventCreateWithFlags(&r->param_ev[i], hipEventDisableTiming), "param event");
-    KT_CREATE_HIP(hipEventCreate(&r->ev_trace0), "trace event");
-    KT_CREATE_HIP(hipEventCreate(&r->ev_trace1), "trace event");
     // frame buffers (init.rs:142-178), allocated for both frames
     const size_t px = (size_t)cfg->width * cfg->height;
     for (auto& f : r->frames) {
@@ -790,8 +810,10 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
 
 void kt_destroy(kt_renderer* r) {
     if (!r) return;
-    if (r->stream) (void)hipStreamSynchronize(r->stream);
-    if (r->last_stream && r->last_stream != r->stream) (void)hipStreamSynchronize(r->last_stream);
+    for (auto& f : r->frames)
+        if (f.stream) (void)hipStreamSynchronize(f.stream);
+    for (auto& sl : r->slots)
+        if (sl.used) (void)hipEventSynchronize(sl.done);
     release(r);
 }
 
@@ -819,23 +841,25 @@ kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* o) {
 
 kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, void* stream) {
     if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
-    return trace_into(r, cam, hdr, cls, r->d_trace_flag, (hipStream_t)stream);
+    return trace_into(r, cam, hdr, cls, nullptr, (hipStream_t)stream);
 }
 
 kt_status kt_trace_ex(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, uint32_t* attempts,
                       void* stream) {
     if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
-    return trace_into(r, cam, hdr, cls, r->d_trace_flag, (hipStream_t)stream, attempts);
+    return trace_into(r, cam, hdr, cls, nullptr, (hipStream_t)stream, attempts);
 }
 
 kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
     if (!r || !out) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
-    KT_HIP(r, hipStreamSynchronize(r->last_stream), "stream sync");
+    if (r->last_slot < 0) return set_err(r, KT_ERR_INVALID_ARGUMENT, "no trace has been issued");
+    const kt_renderer::Slot& sl = r->slots[r->last_slot];
+    KT_HIP(r, hipEventSynchronize(sl.done), "trace sync");
     kt::DevCounters c;
     uint32_t flag[2];
-    KT_HIP(r, hipMemcpy(&c, r->d_counters, sizeof c, hipMemcpyDeviceToHost), "read counters");
-    KT_HIP(r, hipMemcpy(flag, r->d_trace_flag, sizeof flag, hipMemcpyDeviceToHost), "read flag");
-    out->rays = r->last_rays;
+    KT_HIP(r, hipMemcpy(&c, sl.counters, sizeof c, hipMemcpyDeviceToHost), "read counters");
+    KT_HIP(r, hipMemcpy(flag, sl.flag_used, sizeof flag, hipMemcpyDeviceToHost), "read flag");
+    out->rays = sl.rays;
     out->rhs_evals = c.rhs_evals;
     out->attempts = c.attempts;
     out->accepted = c.accepted;
@@ -855,7 +879,7 @@ kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
                        ? (double)(c.clock_end - c.clock_drain) * us_per_tick
                        : 0.0;
     float ms = 0.0f;
-    out->kernel_event_ms = hipEventElapsedTime(&ms, r->ev_trace0, r->ev_trace1) == hipSuccess ? (double)ms : 0.0;
+    out->kernel_event_ms = hipEventElapsedTime(&ms, sl.ev0, sl.done) == hipSuccess ? (double)ms : 0.0;
     return KT_OK;
 }
 
@@ -875,7 +899,11 @@ kt_status kt_submit_render_f64(kt_renderer* r, const kt_camera* cam, int32_t* su
     }
     if (idx < 0) return set_err(r, KT_ERR_DEVICE, "no free frame buffer");
     auto& f = r->frames[idx];
-    hipStream_t s = r->stream;
+    // Each frame gets its own stream, so two submitted frames overlap on the
+    // device.  They are created on first use: a process has few hardware
+    // queues (four by default) and streams beyond that share them.
+    if (!f.stream) KT_HIP(r, hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipStream_t s = f.stream;
     int rc = trace_into(r, cam, f.hdr, nullptr, f.flag, s);  // clears f.flag first
     if (rc != KT_OK) return rc;
     KT_HIP(r, hipMemcpyAsync(f.flag_host, f.flag, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "flag copy");
@@ -1367,7 +1395,7 @@ kt_status kt_autotune(kt_renderer* r, const kt_camera* cam, float* hdr, void* st
         float t_min = 3.0e38f;
         for (int k = 0; k < kTuneReps && rc == KT_OK; ++k) {
             hipError_t e = hipEventRecord(e0, s);
-            if (e == hipSuccess) rc = trace_into(r, cam, hdr, nullptr, r->d_trace_flag, s);
+            if (e == hipSuccess) rc = trace_into(r, cam, hdr, nullptr, nullptr, s);
             if (rc == KT_OK && e == hipSuccess) e = hipEventRecord(e1, s);
             if (rc == KT_OK && e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.0f;

@@ -132,40 +132,46 @@ def render(theta, renderer):
     return _KERR_RENDER.apply(theta, renderer)
 
 
-def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), final_lr=1.0, log_every=0):
+def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), hold=0.6, final_lr=0.1,
+             log_every=0):
     """Fit theta to a target frame with Adam on the L2 loss over the linear
     HDR colour channels.  Adam's step is about its learning rate in each
     coordinate whatever the gradient's scale, so theta is optimised as
-    theta = step * z on z: `step` is the initial per-iteration step size in
-    spin, inclination [rad], r_obs [M] and mdot; the learning rate decays
-    geometrically from 1 to `final_lr` over the run (1: constant, which
-    suits this loss: its frozen-event gradient ignores the image's
-    discontinuities, and a decaying step stalls in the ripples they leave).
-    Returns (theta [4] list, history of (loss, theta) before each update)."""
+    theta = step * z on z: `step` is the per-iteration step size in spin,
+    inclination [rad], r_obs [M] and mdot.  The learning rate stays 1 for
+    the first `hold` fraction of the iterations, then falls linearly to
+    `final_lr`.  The loss is only piecewise smooth (its frozen-event
+    gradient ignores the image's discontinuities, whose ripples make a
+    constant step wander), so the fit returns the iterate of lowest loss.
+    Returns (theta [4] list of the best iterate, history of (loss, theta)
+    before each update)."""
     torch = _torch()
     scale = torch.tensor(step, dtype=torch.float64)
     z = (torch.tensor([float(v) for v in theta0], dtype=torch.float64) / scale).requires_grad_(True)
     opt = torch.optim.Adam([z], lr=1.0)
-    gamma = final_lr ** (1.0 / max(iters - 1, 1))
-    sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=gamma)
+    n_hold = int(hold * iters)
     target = target_hdr[..., :3].detach().to(torch.float64)
     history = []
+    best = (float("inf"), [float(v) for v in theta0])
     for it in range(iters):
+        frac = 0.0 if it < n_hold else (it - n_hold) / max(iters - 1 - n_hold, 1)
+        for group in opt.param_groups:
+            group["lr"] = 1.0 + (final_lr - 1.0) * frac
         opt.zero_grad()
         theta = z * scale
         hdr = render(theta, renderer)
         loss = ((hdr[..., :3].to(torch.float64) - target) ** 2).sum()
         loss.backward()
         history.append((float(loss.detach()), [float(v) for v in theta.detach()]))
+        if history[-1][0] < best[0]:
+            best = history[-1]
         opt.step()
-        sched.step()
         with torch.no_grad():
             z[0].clamp_(0.0, 0.999 / step[0])
             z[3].clamp_(min=1e-3 / step[3])
         if log_every and it % log_every == 0:
             print(f"[adam] it {it} loss {history[-1][0]:.6e} theta {history[-1][1]}", flush=True)
-    theta = (z * scale).detach()
-    return [float(v) for v in theta], history
+    return list(best[1]), history
 
 
 def autotune_checkpoint(cfg, theta, intervals=(8, 16, 32, 64), fov_deg=GRAD_FOV_DEG, log_steps_per_ray=512, reps=2):

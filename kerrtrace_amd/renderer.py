@@ -28,7 +28,9 @@ class FrameView:
 
 
 class HipRenderer:
-    """One renderer handle (one HIP stream, two frame buffers)."""
+    """One renderer handle: two frame buffers with a stream each for the
+    frame API, and stream-ordered traces on caller streams (traces issued on
+    two streams run concurrently)."""
 
     def __init__(self, cfg: KtConfig):
         _torch()
@@ -116,7 +118,7 @@ class HipRenderer:
         return hdr, cls
 
     def stats(self):
-        """Counters of the last trace (synchronises its stream)."""
+        """Counters of the last trace (waits for it to finish)."""
         st = KtTraceStats()
         check(lib().kt_get_trace_stats(self._h, C.byref(st)), self._h)
         return {name: getattr(st, name) for name, _ in st._fields_}

@@ -128,3 +128,20 @@ def test_step_log_overflow_is_reported(torch_gpu):
         with pytest.raises(KerrTraceError) as ei:
             g.backward(torch.zeros((16, 16, 4), dtype=torch.float32, device="cuda"))
         assert ei.value.code == KT_ERR_CAPACITY
+
+
+def test_adam_fit_reduces_loss_and_returns_best_iterate(torch_gpu):
+    torch = torch_gpu
+    from kerrtrace_amd.grad import GradRenderer, fit_adam
+    cfg = _cfg((64, 64))
+    say("[grad] adam 64x64, 40 iterations")
+    with GradRenderer(cfg) as g:
+        target, _ = g.forward(K.GRAD_TARGET)
+        target = target.clone()
+        theta, hist = fit_adam(g, target, K.GRAD_START, iters=40)
+        torch.cuda.synchronize()
+    losses = [h[0] for h in hist]
+    say(f"[grad] adam loss {losses[0]:.4e} -> min {min(losses):.4e}, theta {theta}")
+    assert len(hist) == 40 and all(np.isfinite(losses))
+    assert min(losses) < 0.5 * losses[0]
+    assert theta == hist[int(np.argmin(losses))][1]

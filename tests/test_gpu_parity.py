@@ -163,6 +163,37 @@ def test_frame_api_sequencing(torch_gpu):
         assert ok
 
 
+def test_traces_on_two_streams(torch_gpu):
+    """Consecutive traces alternate between two sets of launch state, so
+    traces issued on two streams run concurrently: every frame of an
+    interleaved run equals the frame traced alone, class buffer included,
+    and the counters read afterwards are those of the last trace."""
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg, cam = K.scaled(cfg, cam, 240, 136)
+    say("[parity] two streams 240x136")
+    with HipRenderer(cfg) as r:
+        ref_hdr, ref_cls = r.trace(cam)
+        torch.cuda.synchronize()
+        ref_st = r.stats()
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        bufs = [(r.alloc_hdr(), torch.empty_like(ref_cls)) for _ in range(3)]
+        for k in range(6):  # three buffers over two streams and two slots
+            hdr, cls = bufs[k % 3]
+            s = streams[k % 2]
+            if k >= 3:
+                s.wait_stream(streams[(k - 3) % 2])  # buffer k % 3 was last written there
+            r.trace(cam, hdr=hdr, cls=cls, stream=s)
+        torch.cuda.synchronize()
+        st = r.stats()
+        for hdr, cls in bufs:
+            assert torch.equal(hdr, ref_hdr)
+            assert torch.equal(cls, ref_cls)
+        for key in ("rays", "attempts", "accepted", "rhs_evals", "crossings", "max_attempts_ray"):
+            assert st[key] == ref_st[key], key
+
+
 def test_lut_overflow_rejects_frame(torch_gpu):
     """A temperature above lut_max_temp fails the frame (kernel.cu:3-10, mod.rs:236-241)."""
     from kerrtrace_amd._lib import KerrTraceError
