@@ -115,9 +115,11 @@ def pipelined_frames(r, cam, bufs, streams, steps):
     return e0.elapsed_time(e1)
 
 
-def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
-    """The same frame through the mixed-precision tracer: speed relative to
-    the FP64 tracer and error against the FP64 frame (per-pixel hit class,
+def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64, launch_opts):
+    """The same frame through the mixed-precision tracer at tolerances 1e-9
+    (the workload's), 1e-8 and 1e-7: its time against the FP64 tracer at the
+    same tolerance (and against the FP64 frame at 1e-9), and its error
+    against the FP64 frame of the same tolerance (per-pixel hit class,
     sub-class, relative HDR error on matching non-boundary pixels)."""
     import numpy as np
     import torch
@@ -125,41 +127,97 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64):
     from kerrtrace_amd import config as K
     from kerrtrace_amd.renderer import HipRenderer
     from tests.parity import compare
-    cls64 = torch.empty((cfg.height, cfg.width), dtype=torch.uint8, device="cuda")
-    r64.trace(cam, hdr=hdr64, cls=cls64)
-    torch.cuda.synchronize()
-    ref_hdr, ref_cls = hdr64.cpu().numpy().astype(np.float64), cls64.cpu().numpy()
     out = {}
     for tol in (cfg.tolerance, 1e-8, 1e-7):
-        c = cfg.copy()
-        c.precision = K.KT_PRECISION_MIXED
-        c.tolerance = tol
-        rm = HipRenderer(c)
-        hm = rm.alloc_hdr()
-        clm = torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda")
-        rm.trace(cam, hdr=hm, cls=clm)
-        torch.cuda.synchronize()
-        t, st = timed_traces(rm, cam, hm, args.steps)
-        rm.trace(cam, hdr=hm, cls=clm)
-        torch.cuda.synchronize()
-        m = compare(hm.cpu().numpy(), clm.cpu().numpy(), ref_hdr, ref_cls)
-        rays = c.width * c.height * c.spp
+        frames, times, stats = {}, {}, {}
+        for prec in (K.KT_PRECISION_FP64, K.KT_PRECISION_MIXED):
+            c = cfg.copy()
+            c.precision = prec
+            c.tolerance = tol
+            if prec == K.KT_PRECISION_FP64 and tol == cfg.tolerance:
+                rr, hh, t, own = r64, hdr64, t64, False
+            else:
+                rr = HipRenderer(c)
+                if launch_opts:
+                    rr.set_launch_opts(**launch_opts)
+                hh, own = rr.alloc_hdr(), True
+                rr.trace(cam, hdr=hh, with_class=False)
+                torch.cuda.synchronize()
+                t, _ = timed_traces(rr, cam, hh, args.steps)
+            cl = torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda")
+            rr.trace(cam, hdr=hh, cls=cl)
+            torch.cuda.synchronize()
+            stats[prec] = rr.stats()
+            frames[prec] = (hh.cpu().numpy().astype(np.float64), cl.cpu().numpy())
+            times[prec] = t
+            if own:
+                rr.close()
+        st = stats[K.KT_PRECISION_MIXED]
+        tm, tf = times[K.KT_PRECISION_MIXED], times[K.KT_PRECISION_FP64]
+        m = compare(*frames[K.KT_PRECISION_MIXED], *frames[K.KT_PRECISION_FP64])
+        rays = cfg.width * cfg.height * cfg.spp
         key = "same_tol" if tol == cfg.tolerance else f"tol_{tol:g}"
         out[key] = {
             "tolerance": tol,
-            "trace_ms": round(t * 1e3, 4),
-            "trace_mrays_s": round(rays / t / 1e6, 4),
-            "speedup_vs_fp64": round(t64 / t, 4),
+            "trace_ms": round(tm * 1e3, 4),
+            "trace_mrays_s": round(rays / tm / 1e6, 4),
+            "fp64_same_tol_trace_ms": round(tf * 1e3, 4),
+            "speedup_vs_fp64_same_tol": round(tf / tm, 4),
+            "speedup_vs_fp64_tol_1e-9": round(t64 / tm, 4),
             "attempts_per_ray": round(st["attempts"] / rays, 3),
+            "fp64_attempts_per_ray": round(stats[K.KT_PRECISION_FP64]["attempts"] / rays, 3),
             "max_attempts_ray": st["max_attempts_ray"],
-            "kernel_ms": round(st["kernel_us"] / 1e3, 4),
-            "tail_ms": round(st["tail_us"] / 1e3, 4),
-            "algorithmic_tflops": round(trace_flops(st, fc) / t / 1e12, 4),
-            "frac_of_fp32_vector_peak": round(trace_flops(st, fc) / t / PEAK_FP32_VECTOR, 4),
-            "vs_fp64_frame": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac", "class_mismatch_non_boundary",
-                                                 "subclass_mismatch", "max_rel_non_boundary", "rms_rel", "p999_rel")},
+            "algorithmic_tflops": round(trace_flops(st, fc) / tm / 1e12, 4),
+            "vs_fp64_frame_same_tol": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac",
+                                                          "class_mismatch_non_boundary", "subclass_mismatch",
+                                                          "max_rel_non_boundary", "rms_rel", "p999_rel")},
         }
-        rm.close()
+    return out
+
+
+def uhd_leg(args, launch_opts):
+    """3840x2160, i = 60 deg, tol 1e-7 at spins 0, 0.5, 0.9 and 0.998: the
+    mixed tracer against the FP64 tracer on the same frame (the FP64 frame
+    is the FP64 oracle's bit for bit), each timed with HIP events around
+    its launches, with the 1080p frame's launch options."""
+    import numpy as np
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.renderer import HipRenderer
+    from tests.parity import compare
+    out = {"workload": "3840x2160, i=60deg, r_obs=100M, fov=20deg, tol=1e-7, FBM 6 octaves, mixed vs FP64"}
+    for spin in (0.0, 0.5, 0.9, 0.998):
+        cfg, cam = K.preset("uhd2160_i60", spin=spin)
+        rays = cfg.width * cfg.height * cfg.spp
+        res = {}
+        for prec in (K.KT_PRECISION_FP64, K.KT_PRECISION_MIXED):
+            c = cfg.copy()
+            c.precision = prec
+            with HipRenderer(c) as r:
+                if launch_opts:
+                    r.set_launch_opts(**launch_opts)
+                h = r.alloc_hdr()
+                cls = torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda")
+                r.trace(cam, hdr=h, cls=cls)
+                torch.cuda.synchronize()
+                t, st = timed_traces(r, cam, h, args.steps)
+                r.trace(cam, hdr=h, cls=cls)
+                torch.cuda.synchronize()
+                res[prec] = (t, st, h.cpu().numpy().astype(np.float64), cls.cpu().numpy())
+        t64, st64, h64, c64 = res[K.KT_PRECISION_FP64]
+        tm, stm, hm, cm = res[K.KT_PRECISION_MIXED]
+        m = compare(hm, cm, h64, c64)
+        out[f"spin_{spin:g}"] = {
+            "fp64_trace_ms": round(t64 * 1e3, 4), "fp64_mrays_s": round(rays / t64 / 1e6, 4),
+            "mixed_trace_ms": round(tm * 1e3, 4), "mixed_mrays_s": round(rays / tm / 1e6, 4),
+            "speedup": round(t64 / tm, 4),
+            "attempts_per_ray": {"fp64": round(st64["attempts"] / rays, 3), "mixed": round(stm["attempts"] / rays, 3)},
+            "vs_fp64_frame": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac", "class_mismatch_non_boundary",
+                                                 "subclass_mismatch", "max_rel", "max_rel_non_boundary", "rms_rel")},
+        }
+        print(f"[bench] 4K spin {spin}: fp64 {t64 * 1e3:.2f} ms, mixed {tm * 1e3:.2f} ms, "
+              f"class mismatches {m['class_mismatch']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -299,6 +357,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-render batch leg")
+    ap.add_argument("--no-uhd", action="store_true", help="skip the 3840x2160 four-spin mixed-vs-FP64 leg")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiable-render leg")
     ap.add_argument("--no-autotune", action="store_true", help="default launch options for the 1080p frame")
     ap.add_argument("--launch-opts", type=str, default=None,
@@ -420,7 +479,10 @@ def main():
     print(f"[bench] timed {args.steps} frames: {latency_ms:.2f} ms/frame one at a time, {ms_per_step:.2f} ms/frame "
           f"streamed (bitwise equal: {same})", file=sys.stderr, flush=True)
     if not args.no_mixed:
-        out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_kernel)
+        out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_kernel, launch_opts)
+    if not args.no_uhd:
+        print("[bench] 4K four-spin leg", file=sys.stderr, flush=True)
+        out["uhd2160_i60"] = uhd_leg(args, launch_opts)
     if not args.no_batch:
         print("[bench] batch leg", file=sys.stderr, flush=True)
         out["batch64"] = {"fp64_tol_1e-9": batch_leg(args, K.KT_PRECISION_FP64, 1e-9),
