@@ -57,7 +57,7 @@ def trace_flops(st, fc):
 def cpu_baseline(cfg, cam, budget_s=20.0):
     """FP64 oracle on a row sample of the frame, all host cores."""
     from oracle import oracle as O
-    threads = os.cpu_count() or 1
+    threads = O.host_threads()
     # probe: a sparse row sample to estimate the per-row cost
     step = max(1, cfg.height // 16)
     _, _, _, st = O.render_rows(cfg, cam, (0, cfg.height, step), threads)
@@ -362,7 +362,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="default launch options for the 1080p frame")
     ap.add_argument("--launch-opts", type=str, default=None,
                     help="fixed launch options for the 1080p frame instead of the autotuner: waves per CU, "
-                         "refill threshold, tile edge, cost order, long-ray attempts (e.g. 4,32,8,1,256)")
+                         "refill threshold, tile edge, cost order, long-ray attempts[, isolate level] "
+                         "(e.g. 6,32,8,1,256,1)")
     ap.add_argument("--adam-iters", type=int, default=200, help="Adam iterations of the fit (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -385,11 +386,11 @@ def main():
     ldr = torch.empty((H, W), dtype=torch.int32, device="cuda")
     if args.launch_opts:
         vals = [int(v) for v in args.launch_opts.split(",")]
-        if len(vals) != 5:
-            sys.exit("--launch-opts takes five integers")
+        if len(vals) not in (5, 6):
+            sys.exit("--launch-opts takes five or six integers")
         r.set_launch_opts(*vals)
-        launch_opts = dict(zip(("waves_per_cu", "refill_threshold", "queue_chunk", "cost_order", "long_ray_attempts"),
-                               vals))
+        launch_opts = dict(zip(("waves_per_cu", "refill_threshold", "queue_chunk", "cost_order", "long_ray_attempts",
+                                "isolate_level"), vals + [0] * (6 - len(vals))))
     else:
         launch_opts = None if args.no_autotune else r.autotune(cam, hdr=hdr)
     for _ in range(args.warmup):

@@ -195,6 +195,10 @@ typedef struct kt_launch_opts {
                                critical curve); 2: row-major order */
     int32_t long_ray_attempts; /* a wave holding a ray past this many attempts
                                raises its issue priority (3x, 8x: higher) */
+    int32_t isolate_level;  /* 0: off; 1-3: a wave whose priority level (above)
+                               reaches this takes no new rays until its long
+                               ray ends, so the ray's serial chain is not
+                               slowed by wave-mates starting and shading */
 } kt_launch_opts;
 
 typedef struct kt_renderer kt_renderer;

@@ -137,6 +137,7 @@ class KtLaunchOpts(C.Structure):
         ("queue_chunk", C.c_int32),
         ("cost_order", C.c_int32),
         ("long_ray_attempts", C.c_int32),
+        ("isolate_level", C.c_int32),
     ]
 
 

@@ -238,7 +238,7 @@ struct kt_renderer {
     hipEvent_t param_ev[kRing] = {};
     bool param_ev_used[kRing] = {};
     int param_next = 0;
-    kt_launch_opts opts{8, 8, 8, 1, 256};
+    kt_launch_opts opts{8, 8, 8, 1, 256, 0};
     // double-buffered frames (mod.rs:24-51, 88-91), each on its own stream
     struct Frame {
         float* hdr = nullptr;
@@ -395,6 +395,7 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     int tile = r->opts.queue_chunk > 0 ? r->opts.queue_chunk : 8;
     make_params(r, r->cfg, r->d, *cam, hdr, cls, flag, tile, P, attempts);
     P.long_ray = r->opts.long_ray_attempts > 0 ? (uint32_t)r->opts.long_ray_attempts : 256u;
+    P.isolate = (uint32_t)r->opts.isolate_level;
     const bool order = r->opts.cost_order != 2 && tile <= 8;
     const int n_tiles = (int)P.n_tiles;
     const uint64_t items = (uint64_t)n_tiles * (uint64_t)tile * (uint64_t)tile;
@@ -463,7 +464,7 @@ struct kt_batch {
     uint32_t* d_hist = nullptr;
     uint32_t* d_perm = nullptr;
     size_t max_tiles = 0;
-    kt_launch_opts opts{8, 8, 8, 1, 256};
+    kt_launch_opts opts{8, 8, 8, 1, 256, 0};
     hipStream_t last_stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_params = nullptr;
     bool params_in_flight = false;
@@ -833,7 +834,8 @@ kt_status kt_get_luts(const kt_renderer* r, double* disk, double* bb) {
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* o) {
     if (!r || !o) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
-        o->queue_chunk < 0 || o->queue_chunk > 256 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0)
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0 ||
+        o->isolate_level < 0 || o->isolate_level > 3)
         return set_err(r, KT_ERR_INVALID_ARGUMENT, "launch option out of range");
     r->opts = *o;
     return KT_OK;
@@ -1062,7 +1064,8 @@ void kt_batch_destroy(kt_batch* b) {
 kt_status kt_batch_set_launch_opts(kt_batch* b, const kt_launch_opts* o) {
     if (!b || !o) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (o->waves_per_cu < 0 || o->waves_per_cu > 32 || o->refill_threshold < 0 || o->refill_threshold > 64 ||
-        o->queue_chunk < 0 || o->queue_chunk > 256 || o->queue_chunk == 1 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0)
+        o->queue_chunk < 0 || o->queue_chunk > 256 || o->queue_chunk == 1 || o->cost_order < 0 || o->cost_order > 2 || o->long_ray_attempts < 0 ||
+        o->isolate_level < 0 || o->isolate_level > 3)
         return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "launch option out of range (batch tile edge >= 2)");
     b->opts = *o;
     return KT_OK;
@@ -1084,6 +1087,7 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr, uint8_t* const* cls, vo
         kt::RenderParams& P = b->h_params[i];
         make_params(nullptr, b->cfgs[i], b->ds[i], b->cams[i], hdr[i], cls ? cls[i] : nullptr, b->d_flag, tile, P);
         P.long_ray = b->opts.long_ray_attempts > 0 ? (uint32_t)b->opts.long_ray_attempts : 256u;
+        P.isolate = (uint32_t)b->opts.isolate_level;
         P.disk_lut = b->d_disk_luts[i];
         int bb = i;
         while (!b->d_bb_luts[bb]) --bb;  // the first render with the same table owns it
@@ -1371,13 +1375,14 @@ kt_status kt_grad_get_stats(kt_grad* g, kt_grad_stats* out) {
 }
 
 // ------------------------------------------------------------- autotuner
-// Candidate launch options: resident waves per CU (one or two per SIMD),
-// idle lanes that trigger a refill (how many rays a wave carries before it
-// takes new work) and the tile edge of the work order.
+// Candidate launch options: resident waves per CU (one or two per SIMD,
+// or both mixed), idle lanes that trigger a refill (how many rays a wave
+// carries before it takes new work), the tile edge of the work order, and
+// whether a wave holding a long ray stops taking new rays.
 static const kt_launch_opts kTuneGrid[] = {
-    {8, 8, 8, 1, 256},  {8, 16, 8, 1, 256}, {8, 32, 8, 1, 256}, {8, 8, 4, 1, 256},
-    {8, 16, 4, 1, 256}, {4, 8, 8, 1, 256},  {4, 16, 8, 1, 256}, {4, 32, 8, 1, 256},
-    {4, 16, 4, 1, 256}, {4, 8, 4, 1, 256},
+    {8, 16, 8, 1, 256, 0}, {8, 32, 8, 1, 256, 0}, {4, 16, 8, 1, 256, 0}, {4, 32, 8, 1, 256, 0},
+    {8, 8, 4, 1, 256, 0},  {4, 16, 4, 1, 256, 0}, {6, 32, 8, 1, 256, 1}, {8, 32, 8, 1, 256, 1},
+    {4, 32, 8, 1, 256, 1}, {6, 16, 8, 1, 256, 1}, {8, 16, 8, 1, 256, 2}, {6, 32, 8, 1, 256, 2},
 };
 static const int kTuneReps = 2;
 

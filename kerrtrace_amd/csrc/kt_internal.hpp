@@ -60,6 +60,7 @@ struct RenderParams {
     int32_t W, H, spp, max_steps, max_attempts, lut_n, noise_detail, disk_root_mode;
     int32_t rhs_exact;        // 1: exact Hamiltonian dr^2/dy
     uint32_t long_ray;        // attempts after which a ray raises its wave's priority
+    uint32_t isolate;         // 0, or the priority level from which a wave takes no new rays
     // work queue: items are (global tile, pixel within the tile); the
     // tiles of all renders of a launch are numbered consecutively
     uint32_t tile_begin;      // first global tile of this render

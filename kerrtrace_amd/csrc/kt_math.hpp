@@ -5,9 +5,11 @@
 // a*b+c is never fused behind the source's back); the only non-elementary
 // trajectory function is pow_m02, written below from its mathematics.  The
 // tracer therefore reproduces any binary64 evaluation of the same operation
-// sequence bit for bit.  Shading-only transcendentals (exp, sin, cos, atan2,
-// asin) come from the device math library and differ from a host libm by at
-// most a few ulp; they never feed back into a trajectory.
+// sequence bit for bit.  The noise's lattice gradients use turn_cos_sin
+// below, written from its mathematics.  The other shading-only
+// transcendentals (exp, sin, cos, atan2, asin) come from the device math
+// library and differ from a host libm by at most a few ulp; they never feed
+// back into a trajectory.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -365,15 +367,47 @@ KT_HDI double disk_root(double y0, double y1, double k1, double k3, double k4, d
 KT_HDI double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 KT_HDI double lerp(double a, double b, double t) { return fma(t, b - a, a); }
 
+// cos and sin of the angle 2 pi hsh / 2^32 (a 32-bit fraction of a turn).
+// The reduction is exact integer arithmetic: q = the nearest quarter turn,
+// rem = hsh - q 2^30 in [-2^29, 2^29), d = rem * (2 pi 2^-32) in [-pi/4, pi/4];
+// then Taylor series of sin and cos at 0 through d^17 and d^16 (the next
+// terms are below 1e-19) in Horner form, and the quarter-turn rotation
+// (exact swaps and negations).  Within 2 ulp of the exact values.
+KT_HDI void turn_cos_sin(uint32_t hsh, double& c, double& s) {
+    const uint32_t q = (hsh + 0x20000000u) >> 30;
+    const int32_t rem = (int32_t)(hsh - (q << 30));
+    const double d = (double)rem * 0x1.921fb54442d18p-30;
+    const double z = d * d;
+    double ps = fma(0x1.952c77030ad4ap-49, z, -0x1.ae7f3e733b81fp-41);
+    ps = fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, z, 0x1.1111111111111p-7);
+    ps = fma(ps, z, -0x1.5555555555555p-3);
+    const double sn = fma(d * z, ps, d);
+    double pc = fma(0x1.ae7f3e733b81fp-45, z, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, z, 0x1.5555555555555p-5);
+    pc = fma(pc, z, -0.5);
+    const double cs = fma(z, pc, 1.0);
+    const double a = (q & 1u) ? sn : cs;  // cos of the rotated angle, up to sign
+    const double b = (q & 1u) ? cs : sn;  // sin ...
+    c = ((q + 1u) & 2u) ? -a : a;         // q = 1, 2: negative cosine
+    s = (q & 2u) ? -b : b;                // q = 2, 3: negative sine
+}
+
 // cuda/kerr_math.cuh:220-260: gradient noise on the integer lattice; the
 // gradient angle of a lattice point comes from pcg32 of its packed
-// coordinates (xor-ed with the seed mix, 0 = reference hash).
+// coordinates (xor-ed with the seed mix, 0 = reference hash) as a fraction
+// hsh / 2^32 of a turn.
 KT_HDI void lattice_gradient(int ix, int iy, uint64_t seed_mix, double& gx, double& gy) {
     uint64_t st = ((uint64_t)(uint32_t)ix << 32) | (uint64_t)(uint32_t)iy;
     st ^= seed_mix;
-    uint32_t hsh = pcg32(st);
-    double angle = (double)hsh * (2.0 * 3.14159265358979323846 * 0x1p-32);
-    sincos(angle, &gy, &gx);
+    turn_cos_sin(pcg32(st), gx, gy);
 }
 KT_HDI double gradient_noise(double u, double v, uint64_t seed_mix) {
     double fu = floor(u), fv = floor(v);

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     bool drained = false;
     int prio = 0;
     const uint32_t long_ray = prm<BATCH>(Ps, 0).long_ray;
+    const int isolate = (int)prm<BATCH>(Ps, 0).isolate;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
     if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
@@ -68,7 +69,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
             end_sample(L, prm<BATCH>(Ps, L.rid), H);
         }
         // ---- refill idle lanes from the work queue (wave-uniform branch)
-        refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
+        // (a wave isolated around a long ray takes no new rays until it ends)
+        if (isolate == 0 || prio < isolate)
+            refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
 
         if (drained && __ballot(L.active) == 0) break;
         // a wave holding a long-running ray (photon-ring orbits) issues

@@ -69,12 +69,14 @@ class HipRenderer:
         return self.cfg.height
 
     # ---- stream-ordered device API
-    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0):
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0,
+                        isolate_level=0):
         """Persistent-tracer knobs (0 = default): resident waves per CU, idle
         lanes that trigger a refill, tile edge of the work order, tile order
         (0/1 hardest-first, 2 row-major), attempts after which a ray raises
-        its wave's priority."""
-        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
+        its wave's priority, and the priority level (1-3) from which such a
+        wave takes no new rays (0: never)."""
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts, isolate_level)
         check(lib().kt_set_launch_opts(self._h, C.byref(o)), self._h)
 
     def autotune(self, cam, hdr=None, stream=None):
@@ -212,8 +214,9 @@ class HipBatch:
     def rays(self):
         return sum(c.width * c.height * c.spp for c in self.cfgs)
 
-    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0):
-        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts)
+    def set_launch_opts(self, waves_per_cu=0, refill_threshold=0, queue_chunk=0, cost_order=0, long_ray_attempts=0,
+                        isolate_level=0):
+        o = KtLaunchOpts(waves_per_cu, refill_threshold, queue_chunk, cost_order, long_ray_attempts, isolate_level)
         check(lib().kt_batch_set_launch_opts(self._h, C.byref(o)))
 
     def autotune(self, hdr=None, stream=None):

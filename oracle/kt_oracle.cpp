@@ -405,14 +405,13 @@ S dense_dt(S y0, S y1, S k1, S k3, S k4, S k5, S k6, S k7, S h, S t) {
 // cuda/kerr_math.cuh:172-179
 template <class S> S fade(S t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 template <class S> S lerp(S a, S b, S t) { return fma_(t, b - a, a); }
-// cuda/kerr_math.cuh:220-234 (seed extension: 0 == reference)
+// cuda/kerr_math.cuh:220-234 (seed extension: 0 == reference).  The
+// gradient angle is the fraction h / 2^32 of a turn; its cosine and sine
+// come from turn_cos_sin (exact reduction, kt_scalar.hpp).
 inline void gradient(int ix, int iy, uint64_t seed, double& gx, double& gy) {
     uint64_t st = ((uint64_t)(uint32_t)ix << 32) | (uint64_t)(uint32_t)iy;
     st ^= seed * 0x9E3779B97F4A7C15ULL;
-    uint32_t h = pcg32(st);
-    double angle = (double)h * (2.0 * kPi * 0x1p-32);
-    gx = std::cos(angle);
-    gy = std::sin(angle);
+    turn_cos_sin(pcg32(st), gx, gy);
 }
 // cuda/kerr_math.cuh:235-260
 template <class S>
@@ -895,6 +894,7 @@ double kto_rand01(uint64_t s) { return rand01(s); }
 double kto_isco(double a_norm, int prograde, double mass) { return calc_isco(a_norm, prograde != 0, mass); }
 double kto_nt_factor(double r, double a_norm, double r_isco, double inv_m) { return nt_factor(r, a_norm, r_isco, inv_m); }
 double kto_pow_m02(double x) { return pow_m02(x); }
+void kto_turn_cos_sin(uint32_t h, double* c, double* s) { turn_cos_sin(h, *c, *s); }
 double kto_ks_r(double x, double y, double z, double a2) { return ks_r(x, y, z, a2); }
 double kto_dense(const double c[8], double h, double t) {
     return dense(c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], h, t);

@@ -71,6 +71,39 @@ inline double pow_m02(double x) {
     return std::ldexp(y * two_m_j5[j], -q);
 }
 
+// cos and sin of 2 pi h / 2^32 (h a 32-bit fraction of a turn): exact
+// integer reduction to the nearest quarter turn, d = rem * (2 pi 2^-32) in
+// [-pi/4, pi/4], Taylor series of sin through d^17 and of cos through d^16
+// (remainders < 1e-19), exact quarter-turn rotation.
+inline void turn_cos_sin(uint32_t h, double& c, double& s) {
+    const uint32_t q = (h + 0x20000000u) >> 30;
+    const int32_t rem = (int32_t)(h - (q << 30));
+    const double d = (double)rem * 0x1.921fb54442d18p-30;  // 2 pi / 2^32
+    const double z = d * d;
+    // (-1)^k / (2k+1)!, k = 8..1
+    double ps = std::fma(0x1.952c77030ad4ap-49, z, -0x1.ae7f3e733b81fp-41);
+    ps = std::fma(ps, z, 0x1.6124613a86d09p-33);
+    ps = std::fma(ps, z, -0x1.ae64567f544e4p-26);
+    ps = std::fma(ps, z, 0x1.71de3a556c734p-19);
+    ps = std::fma(ps, z, -0x1.a01a01a01a01ap-13);
+    ps = std::fma(ps, z, 0x1.1111111111111p-7);
+    ps = std::fma(ps, z, -0x1.5555555555555p-3);
+    const double sn = std::fma(d * z, ps, d);
+    // (-1)^k / (2k)!, k = 8..1
+    double pc = std::fma(0x1.ae7f3e733b81fp-45, z, -0x1.93974a8c07c9dp-37);
+    pc = std::fma(pc, z, 0x1.1eed8eff8d898p-29);
+    pc = std::fma(pc, z, -0x1.27e4fb7789f5cp-22);
+    pc = std::fma(pc, z, 0x1.a01a01a01a01ap-16);
+    pc = std::fma(pc, z, -0x1.6c16c16c16c17p-10);
+    pc = std::fma(pc, z, 0x1.5555555555555p-5);
+    pc = std::fma(pc, z, -0.5);
+    const double cs = std::fma(z, pc, 1.0);
+    const double a = (q & 1u) ? sn : cs;
+    const double b = (q & 1u) ? cs : sn;
+    c = ((q + 1u) & 2u) ? -a : a;
+    s = (q & 2u) ? -b : b;
+}
+
 // ------------------------------------------------------------ dual number
 template <int N>
 struct Dual {

@@ -45,6 +45,7 @@ def lib():
         L.kto_isco.argtypes, L.kto_isco.restype = [d, i32, d], d
         L.kto_nt_factor.argtypes, L.kto_nt_factor.restype = [d, d, d, d], d
         L.kto_pow_m02.argtypes, L.kto_pow_m02.restype = [d], d
+        L.kto_turn_cos_sin.argtypes, L.kto_turn_cos_sin.restype = [u32, pd, pd], None
         L.kto_ks_r.argtypes, L.kto_ks_r.restype = [d, d, d, d], d
         L.kto_dense.argtypes, L.kto_dense.restype = [pd, d, d], d
         L.kto_dense_dt.argtypes, L.kto_dense_dt.restype = [pd, d, d], d
@@ -113,6 +114,13 @@ def pow_m02(x):
     return lib().kto_pow_m02(x)
 
 
+def turn_cos_sin(h):
+    """(cos, sin) of the angle 2 pi h / 2^32, h a uint32."""
+    c, s = C.c_double(), C.c_double()
+    lib().kto_turn_cos_sin(int(h) & 0xFFFFFFFF, C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
 def ks_r(x, y, z, a2):
     return lib().kto_ks_r(x, y, z, a2)
 
@@ -158,7 +166,8 @@ def render_grad(cfg, theta, fov_deg, block=None, threads=0):
     th = np.asarray(theta, dtype=np.float64)
     hdr = np.zeros((h, w, 3)); tan = np.zeros((h, w, 3, 4))
     cls = np.zeros((h, w), dtype=np.uint8); ns = np.zeros((h, w), dtype=np.uint32)
-    rc = lib().kto_render_grad(C.byref(cfg), _dptr(th), fov_deg, x0, y0, w, h, threads, _dptr(hdr), _dptr(tan),
+    rc = lib().kto_render_grad(C.byref(cfg), _dptr(th), fov_deg, x0, y0, w, h, threads if threads > 0 else host_threads(),
+                               _dptr(hdr), _dptr(tan),
                                cls.ctypes.data_as(C.POINTER(C.c_uint8)), ns.ctypes.data_as(C.POINTER(C.c_uint32)))
     _check(rc)
     return hdr, tan, cls, ns
@@ -171,7 +180,8 @@ def render_frozen(cfg, theta0, theta1, fov_deg, block=None, threads=0):
     x0, y0, w, h = _block(cfg, block)
     t0 = np.asarray(theta0, dtype=np.float64); t1 = np.asarray(theta1, dtype=np.float64)
     h0 = np.zeros((h, w, 3)); h1 = np.zeros((h, w, 3)); same = np.zeros((h, w), dtype=np.uint8)
-    rc = lib().kto_render_frozen(C.byref(cfg), _dptr(t0), _dptr(t1), fov_deg, x0, y0, w, h, threads, _dptr(h0),
+    rc = lib().kto_render_frozen(C.byref(cfg), _dptr(t0), _dptr(t1), fov_deg, x0, y0, w, h,
+                                 threads if threads > 0 else host_threads(), _dptr(h0),
                                  _dptr(h1), same.ctypes.data_as(C.POINTER(C.c_uint8)))
     _check(rc)
     return h0, h1, same.astype(bool)
@@ -210,10 +220,25 @@ def init_ray(cfg, cam_pos, direction):
     return s, pt.value, kt.value
 
 
+MAX_THREADS = 16
+
+
+def host_threads():
+    """Worker threads for the oracle: the CPUs this process may run on,
+    at most MAX_THREADS (a GPU host's share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(MAX_THREADS, n))
+
+
 def render_rows(cfg, cam, rows=None, threads=0):
-    """Trace whole rows.  rows: None (all), or (begin, end, step).
-    Returns (hdr[nrows, W, 3] float64, cls[nrows, W] uint8,
-    attempts[nrows, W] uint32, stats)."""
+    """Trace whole rows.  rows: None (all), or (begin, end, step);
+    threads <= 0: host_threads().  Returns (hdr[nrows, W, 3] float64,
+    cls[nrows, W] uint8, attempts[nrows, W] uint32, stats)."""
+    if threads <= 0:
+        threads = host_threads()
     if rows is None:
         rows = (0, cfg.height, 1)
     b, e, s = rows

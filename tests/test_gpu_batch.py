@@ -59,7 +59,7 @@ def test_batch_matches_single_renders(torch_gpu, precision):
     say(f"[batch] ok: {st['rays']} rays, {st['attempts']} attempts")
 
 
-@pytest.mark.parametrize("opts", [(8, 8, 8, 2), (4, 16, 4, 1), (16, 1, 2, 0)])
+@pytest.mark.parametrize("opts", [(8, 8, 8, 2), (4, 16, 4, 1), (16, 1, 2, 0), (6, 32, 8, 1, 8, 1), (8, 16, 8, 1, 4, 2)])
 def test_batch_launch_options_do_not_change_frames(torch_gpu, opts):
     from kerrtrace_amd.renderer import HipBatch
     renders = _renders(5, 64)
@@ -111,7 +111,8 @@ def test_autotune_keeps_frames_identical(torch_gpu):
         ref = ref.cpu().numpy()
         opts = r.autotune(cam)
         say(f"[autotune] single: {opts}")
-        assert opts["waves_per_cu"] in (4, 8) and opts["refill_threshold"] in (8, 16, 32)
+        assert opts["waves_per_cu"] in (4, 6, 8) and opts["refill_threshold"] in (8, 16, 32)
+        assert opts["isolate_level"] in (0, 1, 2)
         hdr, _ = r.trace(cam)
         assert np.array_equal(hdr.cpu().numpy().view(np.uint32), ref.view(np.uint32))
     renders = _renders(4, 48)

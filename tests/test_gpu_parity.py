@@ -53,6 +53,27 @@ def test_fp64_frame_matches_oracle(torch_gpu, name, size):
     assert np.all(g_hdr[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("name,spin", [("hd1080_a0998", None), ("uhd2160_i60", 0.998)])
+def test_fp64_full_size_frame_matches_oracle(torch_gpu, name, spin):
+    """The measured frames at their full size (1920x1080 at tol 1e-9;
+    3840x2160 at i = 60 deg, tol 1e-7, in FP64): the oracle finishes them in
+    seconds on the host's threads.  The frames are the oracle's bit for bit:
+    every hit class, every attempt count and every HDR float."""
+    cfg, cam = K.preset(name) if spin is None else K.preset(name, spin=spin)
+    cfg.precision = K.KT_PRECISION_FP64
+    say(f"[parity] fp64 full size {name} {cfg.width}x{cfg.height}")
+    g_hdr, g_cls, st = _render_gpu(torch_gpu, cfg, cam)
+    o_hdr, o_cls, _, ost = O.render(cfg, cam)
+    m = compare(g_hdr, g_cls, o_hdr, o_cls)
+    say(f"[parity] {name} full: {m}; gpu attempts {st['attempts']} oracle {ost.attempts} "
+        f"({ost.seconds:.1f} s on {ost.threads} threads)")
+    assert m["class_mismatch"] == 0 and m["subclass_mismatch"] == 0
+    assert m["exact_fp32_frac"] == 1.0
+    assert st["attempts"] == ost.attempts and st["crossings"] == ost.crossings
+    assert st["rhs_evals"] == ost.rhs_evals and st["accepted"] == ost.accepted
+    assert np.all(g_hdr[..., 3] == 1.0)
+
+
 def test_secant_reference_root_mode(torch_gpu):
     """The as-written crossing (clamped secant, tol floor 0) on the 256^2 scene."""
     cfg, cam = K.preset("cpu256_a09")
