@@ -15,7 +15,8 @@ gives the same frames one at a time (`frame_latency_ms`), `trace_mrays_s`
 the tracer alone.  `roofline` prices the tracer's
 algorithmic FP64 FLOPs (exact per-unit counts from
 tests/golden/flop_constants.json times the work counters the kernel writes)
-against the FP64 vector peak; `post_roofline` prices bloom + display
+against the FP64 vector peak, over the tracer launch alone (`achieved`) and
+over the streamed frame time (`streamed_achieved`); `post_roofline` prices bloom + display
 against HBM bandwidth.  `cpu_baseline` times the FP64 CPU oracle on a
 bounded row sample of the same frame on the host cores.
 """
@@ -472,7 +473,9 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch"),
                      "kernel": "trace_fp64_persistent",
-                     "flop_per_frame": flops},
+                     "flop_per_frame": flops,
+                     "streamed_achieved": round(flops / (ms_per_step / 1e3) / 1e12, 4),
+                     "streamed_frac": round(flops / (ms_per_step / 1e3) / PEAK_FP64_VECTOR, 4)},
         "post_roofline": {"bound": "hbm", "achieved": round(post_bytes / t_post / 1e9, 2), "peak": PEAK_HBM / 1e9,
                           "unit": "GB/s", "frac": round(post_bytes / t_post / PEAK_HBM, 4), "traffic": None,
                           "kernels": "bloom_h + post_v"},
