@@ -242,3 +242,35 @@ def test_lut_overflow_rejects_frame(torch_gpu):
         st = r.stats()
         assert st["lut_overflow"] == 1 and st["overflow_temperature"] > 2000.0
         assert (cls.cpu().numpy() & 16).any()
+
+
+def test_first_frame_png_from_frame_api(torch_gpu, tmp_path):
+    """src/app.rs:219-237: the first ready frame of the frame loop saved as
+    an RGBA PNG equals trace + post of the same camera, alpha 255."""
+    from kerrtrace_amd import export as E
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("cpu256_a09")
+    cfg, cam = K.scaled(cfg, cam, 96, 64)
+    say("[parity] first-frame png 96x64")
+    path = tmp_path / "picture" / "first_frame.png"
+    with HipRenderer(cfg) as r:
+        assert r.submit_render_camera(cam)
+        import time
+        f = None
+        for _ in range(2000):
+            f = r.ready_frame()
+            if f is not None:
+                break
+            time.sleep(0.005)
+        assert f is not None
+        assert E.save_first_frame(str(path), f.data)
+        r.finish_frame(f.index)
+        hdr, _ = r.trace(cam, with_class=False)
+        ldr = r.post_process(hdr)
+        torch_gpu.cuda.synchronize()
+        direct = ldr.cpu().numpy()
+    assert not E.save_first_frame(str(path), direct)  # an existing file is kept
+    from PIL import Image
+    img = np.asarray(Image.open(str(path)))
+    assert img.shape == (64, 96, 4) and np.all(img[..., 3] == 255)
+    assert np.array_equal(img, E.ldr_to_rgba(direct))
