@@ -274,3 +274,31 @@ def test_first_frame_png_from_frame_api(torch_gpu, tmp_path):
     img = np.asarray(Image.open(str(path)))
     assert img.shape == (64, 96, 4) and np.all(img[..., 3] == 255)
     assert np.array_equal(img, E.ldr_to_rgba(direct))
+
+
+@pytest.mark.parametrize("size", [(1, 1), (1, 37), (37, 1), (13, 7), (65, 3)])
+def test_tiny_and_ragged_frames(torch_gpu, size):
+    """Frames smaller than a tile and sizes that are no tile multiple, with
+    tile edges 1, 8 and 16: every pixel traced once, bitwise equal to the
+    oracle; the post-process with a bloom radius larger than the frame."""
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg, cam = K.scaled(cfg, cam, *size)
+    say(f"[parity] ragged {size}")
+    o_hdr, o_cls, o_att, ost = O.render(cfg, cam)
+    with HipRenderer(cfg) as r:
+        for chunk in (1, 8, 16):
+            r.set_launch_opts(waves_per_cu=4, refill_threshold=8, queue_chunk=chunk, cost_order=1)
+            att = torch_gpu.zeros(size[::-1], dtype=torch_gpu.int32, device="cuda")
+            hdr, cls = r.trace(cam, attempts=att)
+            ldr = r.post_process(hdr)
+            torch_gpu.cuda.synchronize()
+            g_hdr, g_cls = hdr.cpu().numpy(), cls.cpu().numpy()
+            m = compare(g_hdr, g_cls, o_hdr, o_cls)
+            assert m["class_mismatch"] == 0 and m["exact_fp32_frac"] == 1.0, (chunk, m)
+            assert np.array_equal(att.cpu().numpy().astype(np.uint32), o_att), chunk
+            st = r.stats()
+            assert st["rays"] == size[0] * size[1] and st["attempts"] == ost.attempts
+            o_ldr, _ = O.post(cfg, g_hdr)
+            d, _ = ldr_diff(ldr.cpu().numpy(), o_ldr)
+            assert d <= 1
