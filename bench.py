@@ -469,7 +469,9 @@ def main():
                  "crossings": st["crossings"], "max_attempts_ray": st["max_attempts_ray"],
                  "attempts_per_ray": round(st["attempts"] / rays, 3),
                  "kernel_ms": round(st["kernel_us"] / 1e3, 4), "tail_ms": round(st["tail_us"] / 1e3, 4)},
-        "roofline": {"bound": "valu-fp64", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
+        "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
+                     "compute_unit": "FP64 VALU (the tracer has no matrix-shaped work and issues no MFMA; MI355X's "
+                                     "FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)",
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch"),
                      "kernel": "trace_fp64_persistent",
