@@ -63,7 +63,7 @@ __device__ __forceinline__ double step_tolerance(const V6& s, double r_c, const 
                                                  double tol_floor) {
     double r2_c = r_c * r_c;
     double dH = fma(r2_c, r2_c, g.aa * s.y * s.y);
-    double Hc = g.M * r_c * r2_c * (1.0 / dH);
+    double Hc = g.M * r_c * r2_c * div_pos(1.0, dH);
     double tq = (Hc - 0.05) / (0.3 - 0.05);
     tq = fmin(fmax(tq, 0.0), 1.0);
     return tol * fmax(5.0 - 5.0 * tq, tol_floor);
@@ -108,6 +108,11 @@ __device__ __forceinline__ double disk_factor(const RenderParams& P, double u) {
 }
 
 // Emission of one equatorial crossing (cuda/kerr_trace.cuh:174-226).  The
+// square roots and quotients whose operands are positive by construction
+// (a hit on the disk lies outside the ISCO, so r > |a|, r^2 + a^2 > 0, the
+// disk span and LUT step are positive) use the guard-free cores; the
+// others (signed angular momentum, the redshift factor) keep the library
+// division.  The
 // lane's colour and transmittance stay in the caller's registers: the
 // function returns the colour to add, d_col * alpha * Tr, and the factor
 // 1 - alpha the caller multiplies into Tr (passing lane state by pointer
@@ -124,26 +129,26 @@ static __device__ __noinline__ Shade shade_crossing(const RenderParams& P, doubl
     const double a = P.a, M = P.M, aa = P.aa;
     double r_hit = ks_r(x_hit, 0.0, z_hit, aa);
     if (!(r_hit >= P.disk_inner && r_hit <= P.disk_outer)) return o;
-    double sqrt_M = sqrt(M), r_sqrt = sqrt(r_hit);
+    double sqrt_M = sqrt_pos(M), r_sqrt = sqrt_pos(r_hit);
     double disc_denom = 1.0 - 3.0 * M / r_hit + 2.0 * a * sqrt_M / (r_hit * r_sqrt);
     if (!(disc_denom > 0.0)) return o;
     double omega = -sqrt_M / (r_hit * r_sqrt + a * sqrt_M);
     double r2_hit = r_hit * r_hit;
     double denom_hit = r2_hit + aa;
-    double inv_dh = 1.0 / denom_hit;
+    double inv_dh = div_pos(1.0, denom_hit);
     double hlx = (r_hit * x_hit - a * z_hit) * inv_dh;
     double hly = 0.0;
     double hlz = (r_hit * z_hit + a * x_hit) * inv_dh;
     double ldotp = fma(hlx, px_hit, fma(hly, py_hit, hlz * pz_hit));
     double ldotxi = hlx * z_hit - hlz * x_hit;
     double dHh = r2_hit * r2_hit;
-    double Hh = M * r_hit * r2_hit * (1.0 / dHh);
+    double Hh = M * r_hit * r2_hit * div_pos(1.0, dHh);
     double L = (z_hit * px_hit - x_hit * pz_hit) + 2.0 * Hh * ldotxi * (ldotp + kt);
     double energy = -pt;
     double l_over_e = L / energy;
-    double g = 1.0 / ((1.0 / sqrt(disc_denom)) * (1.0 - omega * l_over_e));
+    double g = 1.0 / (div_pos(1.0, sqrt_pos(disc_denom)) * (1.0 - omega * l_over_e));
     double dspan = P.disk_outer - P.disk_inner;
-    double uu = dspan > 0.0 ? (r_hit - P.disk_inner) / dspan : 0.0;
+    double uu = dspan > 0.0 ? div_pos(r_hit - P.disk_inner, dspan) : 0.0;
     double sampled = disk_factor(P, uu);
     double T = P.T_scale * sampled * g;
     if (P.noise_detail > 0) {
@@ -152,22 +157,22 @@ static __device__ __noinline__ Shade shade_crossing(const RenderParams& P, doubl
         double rho2 = fma(x_hit, x_hit, fma(0.0, 0.0, z_hit * z_hit));
         double u = rho2 - aa;
         double s_sq = fma(u, u, 4.0 * aa * 0.0 * 0.0);
-        double inv_s = 1.0 / sqrt(s_sq);
+        double inv_s = div_pos(1.0, sqrt_pos(s_sq));
         double s = s_sq * inv_s;
         double r2 = 0.5 * (u + s);
-        double inv_r = 1.0 / sqrt(r2);
+        double inv_r = div_pos(1.0, sqrt_pos(r2));
         r_hit = r2 * inv_r;
         double cos_th = 0.0 * inv_r;
         double cos2 = cos_th * cos_th;
         double sin_sq = 1.0 - cos2;
-        double inv_sin = 1.0 / sqrt(sin_sq);
+        double inv_sin = div_pos(1.0, sqrt_pos(sin_sq));
         double denom = r2 + aa;
-        double inv = inv_sin * (1.0 / denom);
+        double inv = inv_sin * div_pos(1.0, denom);
         double cos_phi = (r_hit * z_hit + a * x_hit) * inv;
         double sin_phi = (r_hit * x_hit - a * z_hit) * inv;
         double phi = atan2(sin_phi, cos_phi);
         double r3 = r_hit * r_hit * r_hit;
-        double om = 1.0 / (a + sqrt(r3 * P.inv_M));
+        double om = div_pos(1.0, a + sqrt_pos(r3 * P.inv_M));
         double phase = phi + P.noise_winding * om;
         double sn, cs;
         sincos(phase, &sn, &cs);
@@ -183,7 +188,7 @@ static __device__ __noinline__ Shade shade_crossing(const RenderParams& P, doubl
     blackbody(P, T, d_col, ovf);
     if (ovf) o.overflow_T = T;
     double luma = 0.2126 * d_col[0] + 0.7152 * d_col[1] + 0.0722 * d_col[2];
-    double luma_scaled = luma / P.wl_step;
+    double luma_scaled = div_pos(luma, P.wl_step);
     double alpha = 1.0 - exp(-luma_scaled * 0.5);
     o.r = d_col[0] * alpha * Tr;
     o.g = d_col[1] * alpha * Tr;
