@@ -371,7 +371,9 @@ void make_params(const kt_renderer* r, const kt_config& c, const Derived& d, con
 }
 
 // Stages a parameter record through the pinned ring and returns its
-// device address (stream-ordered).
+// device address (stream-ordered).  The slot's event is recorded after the
+// copy here and again by the caller after the kernels that read the record;
+// a slot is reused only once that event has completed.
 int upload_params(kt_renderer* r, const kt::RenderParams& P, hipStream_t s, kt::RenderParams** out) {
     int i = r->param_next;
     r->param_next = (i + 1) % kt_renderer::kRing;
@@ -417,6 +419,10 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
                        : kt::launch_trace_fp64(dp, 1, items, thr, sl.counters, blocks, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
     KT_HIP(r, hipEventRecord(sl.done, s), "trace event");
+    // the ring slot's event now follows the kernels that read the record, so
+    // the record is not overwritten under a trace still running on another
+    // stream when the ring comes round
+    KT_HIP(r, hipEventRecord(r->param_ev[dp - r->d_params], s), "param event");
     sl.used = true;
     sl.flag_used = flag;
     sl.rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
@@ -1075,10 +1081,13 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr, uint8_t* const* cls, vo
     if (!b || !hdr) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t s = (hipStream_t)stream;
     const int tile = b->opts.queue_chunk > 0 ? b->opts.queue_chunk : 8;
-    // the pinned records of the previous call must have reached the device
+    // the pinned records of the previous call must have reached the device,
+    // and its launch (on any stream) must be done with the device records,
+    // counters and tile order this call rewrites
     if (b->params_in_flight) {
         hipError_t e = hipEventSynchronize(b->ev_params);
-        if (e != hipSuccess) return hip_err(nullptr, e, "param upload wait");
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, b->ev1, 0);
+        if (e != hipSuccess) return hip_err(nullptr, e, "previous trace wait");
     }
     uint32_t tile_begin = 0;
     uint64_t rays = 0;
@@ -1324,8 +1333,11 @@ kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad, doubl
     if (!g || !dl_dhdr || !grad) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
     if (!g->have_forward) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "kt_grad_backward before kt_grad_forward");
     hipStream_t s = (hipStream_t)stream;
+    // the forward pass may have run on another stream: order after it
+    hipError_t e = hipStreamWaitEvent(s, g->ev[1], 0);
+    if (e != hipSuccess) return hip_err(nullptr, e, "forward wait");
     // a step log that ran out of room cannot be reversed
-    hipError_t e = hipMemcpyAsync(g->h_pool, g->d_pool, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(g->h_pool, g->d_pool, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_err(nullptr, e, "step log check");
     if (g->h_pool[1] != 0u)

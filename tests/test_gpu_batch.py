@@ -124,3 +124,25 @@ def test_autotune_keeps_frames_identical(torch_gpu):
         hdr, _ = b.trace()
         for k in range(len(renders)):
             assert np.array_equal(hdr[k].cpu().numpy().view(np.uint32), ref[k].view(np.uint32))
+
+
+def test_batch_traces_on_two_streams(torch_gpu):
+    """Consecutive batch traces on different streams are ordered by the
+    batch itself (they share its device records and counters)."""
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipBatch
+    renders = _renders(4, 48)
+    say("[batch] two streams")
+    with HipBatch(renders) as b:
+        ref, _ = b.trace()
+        ref = [t.cpu().numpy() for t in ref]
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        outs = []
+        for k in range(4):
+            hdr, _ = b.alloc(with_class=False)
+            b.trace(hdr=hdr, with_class=False, stream=s1 if k % 2 == 0 else s2)
+            outs.append(hdr)
+        torch.cuda.synchronize()
+        for hdr in outs:
+            for k in range(len(renders)):
+                assert np.array_equal(hdr[k].cpu().numpy().view(np.uint32), ref[k].view(np.uint32)), k

@@ -145,3 +145,24 @@ def test_adam_fit_reduces_loss_and_returns_best_iterate(torch_gpu):
     assert len(hist) == 40 and all(np.isfinite(losses))
     assert min(losses) < 0.5 * losses[0]
     assert theta == hist[int(np.argmin(losses))][1]
+
+
+def test_backward_on_another_stream(torch_gpu):
+    """kt_grad_backward orders itself after the forward pass even when the
+    caller runs them on different streams."""
+    torch = torch_gpu
+    from kerrtrace_amd.grad import GradRenderer
+    cfg = _cfg((48, 40))
+    w = torch.from_numpy(_weights((cfg.height, cfg.width), 5)).cuda()
+    say("[grad] forward and backward on two streams")
+    with GradRenderer(cfg) as g:
+        g.forward(THETA)
+        ref = g.backward(w).cpu()
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        w2 = w.clone()
+        torch.cuda.synchronize()
+        for _ in range(3):
+            g.forward(THETA, stream=s1)
+            got = g.backward(w2, stream=s2)
+            s2.synchronize()
+            assert torch.equal(got.cpu(), ref)
