@@ -29,9 +29,11 @@
 namespace kt {
 
 // Per-lane scratch: element j of lane `tid` at sc[j * stride].  Step u of
-// the segment being reversed holds its start state at j = 36 u and the
-// inputs of its stages 2..6 at j = 36 u + 6 .. 36 u + 35; the state after
-// the segment's last step sits at j = 36 K.
+// the segment being reversed holds its start state at j = 6 u; the state
+// after the segment's last step sits at j = 6 K.  The stage inputs of a
+// step are formed again from its start state when the step is reversed:
+// five more right-hand sides per step instead of 30 more doubles of
+// scratch traffic each way.
 struct LaneScratch {
     double* p;
     uint32_t stride;
@@ -113,26 +115,25 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
     uint32_t chunk = LG.head[pix];
     if (chunk >= LG.capacity) return;
     uint32_t seg_start = ((N - 1) / K) * K;
-    const uint32_t j_end = 36 * K;
+    const uint32_t j_end = 6 * K;
     for (;;) {
         const double* ch = LG.arena + (size_t)chunk * LG.stride;
         const uint32_t len = min(K, N - seg_start);
         // re-integrate the segment from its checkpoint with the logged steps
         V6 s{ch[0], ch[1], ch[2], ch[3], ch[4], ch[5]};
         for (uint32_t u = 0; u < len; ++u) {
-            V6 Y[5];
-            sc.put6(36 * u, s);
-            s = dp_step(s, ch[8 + u], pt, g, Y).ns;
-            for (int i = 0; i < 5; ++i) sc.put6(36 * u + 6 + 6 * i, Y[i]);
+            sc.put6(6 * u, s);
+            s = dp_step(s, ch[8 + u], pt, g, nullptr).ns;
         }
         sc.put6(j_end, s);
         for (int u = (int)len - 1; u >= 0; --u) {
-            const V6 sn = sc.get6(36 * (uint32_t)u);
-            const V6 ns = sc.get6((uint32_t)u + 1 < len ? 36 * ((uint32_t)u + 1) : j_end);
+            const V6 sn = sc.get6(6 * (uint32_t)u);
+            const V6 ns = sc.get6((uint32_t)u + 1 < len ? 6 * ((uint32_t)u + 1) : j_end);
             const double hs = ch[8 + u];
             const double Tr_n = ch[8 + K + u];
+            // the step's stage inputs, formed as the forward pass formed them
             V6 Y[5];
-            for (int i = 0; i < 5; ++i) Y[i] = sc.get6(36 * (uint32_t)u + 6 + 6 * i);
+            (void)dp_step(sn, hs, pt, g, Y);
             const double r_now = ks_r(ns.x, ns.y, ns.z, P.aa);
             const bool horizon = r_now < P.rh_eps;
             const bool crossing = !horizon && ((sn.y < 0.0 && ns.y >= 0.0) || (sn.y > 0.0 && ns.y <= 0.0));

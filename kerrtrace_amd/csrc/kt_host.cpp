@@ -1236,7 +1236,7 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
     KT_GRAD_HIP(hipMalloc(&g->d_pool, 2 * sizeof(uint32_t)), "hipMalloc pool head");
     KT_GRAD_HIP(hipMalloc(&g->d_grad_px, 4 * n_pix * sizeof(double)), "hipMalloc per-pixel gradient");
     g->threads = (uint32_t)std::max(256, (g->cus * 8 + 3) / 4 * 256);
-    const size_t sc = (size_t)(36 * g->K + 6) * g->threads;
+    const size_t sc = (size_t)(6 * g->K + 6) * g->threads;
     KT_GRAD_HIP(hipMalloc(&g->d_scratch, sc * sizeof(double)), "hipMalloc adjoint scratch");
     KT_GRAD_HIP(hipMalloc(&g->d_queue, sizeof(unsigned long long)), "hipMalloc queue");
     KT_GRAD_HIP(hipMalloc(&g->d_rkeys, n_pix), "hipMalloc adjoint keys");
