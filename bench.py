@@ -10,7 +10,8 @@ Prints one JSON line.  `value` = rays per second over a run of whole frames
 (Mrays/s) issued as a renderer streams them: consecutive frames alternate
 between two HIP streams with their own HDR, bloom and image buffers, so up to
 two frames are in flight (as in the reference's double-buffered frame loop)
-and one frame's long-ray tail overlaps the next frame's bulk.  `serial`
+and one frame's long-ray tail overlaps the next frame's bulk; their launch
+options are tuned for that (`pipelined.launch_opts`).  `serial`
 gives the same frames one at a time (`frame_latency_ms`), `trace_mrays_s`
 the tracer alone.  `roofline` prices the tracer's
 algorithmic FP64 FLOPs (exact per-unit counts from
@@ -114,6 +115,27 @@ def pipelined_frames(r, cam, bufs, streams, steps):
     e1.record(cur)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1)
+
+
+STREAM_CANDIDATES = [(w, t, 8, 1, 256, 1) for w in (6, 7, 8) for t in (16, 24, 32)]
+
+
+def tune_streamed(r, cam, bufs, streams, first, frames=6):
+    """Launch options for streamed frames: the frame time of two frames in
+    flight favours other options than a frame alone (more resident waves,
+    rays kept longer per wave).  Tries `first` and STREAM_CANDIDATES, each on
+    `frames` streamed frames, and returns the fastest as a dict."""
+    keys = ("waves_per_cu", "refill_threshold", "queue_chunk", "cost_order", "long_ray_attempts", "isolate_level")
+    cands = [tuple(first[k] for k in keys)] if first else []
+    cands += [c for c in STREAM_CANDIDATES if c not in cands]
+    best, best_ms = None, float("inf")
+    for c in cands:
+        r.set_launch_opts(*c)
+        pipelined_frames(r, cam, bufs, streams, 2)
+        ms = pipelined_frames(r, cam, bufs, streams, frames) / frames
+        if ms < best_ms:
+            best, best_ms = c, ms
+    return dict(zip(keys, best))
 
 
 def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64, launch_opts):
@@ -423,9 +445,14 @@ def main():
     latency_ms = sum(frame_ms) / args.steps
     serial_hdr, serial_ldr = hdr.clone(), ldr.clone()
 
-    # the same frames streamed: two streams, two frames in flight
+    # the same frames streamed: two streams, two frames in flight, with
+    # launch options tuned for streaming (unless they were given)
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]
     bufs = [(hdr, bloom, ldr), (r.alloc_hdr(), torch.empty_like(bloom), torch.empty_like(ldr))]
+    stream_opts = launch_opts
+    if not args.launch_opts and not args.no_autotune:
+        stream_opts = tune_streamed(r, cam, bufs, streams, launch_opts)
+        r.set_launch_opts(**stream_opts)
     pipelined_frames(r, cam, bufs, streams, 2 * max(1, args.warmup))
     print("[bench] pipelined warmup done", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
@@ -433,6 +460,8 @@ def main():
     wall += time.perf_counter() - t1
     ms_per_step = total_ms / args.steps
     same = all(torch.equal(b[0], serial_hdr) and torch.equal(b[2], serial_ldr) for b in bufs)
+    if launch_opts:
+        r.set_launch_opts(**launch_opts)  # the later legs reuse the single-frame options
     t_trace = sum(trace_ms) / len(trace_ms) / 1e3
     t_kernel = sum(kernel_ms) / len(kernel_ms) / 1e3
     t_post = sum(post_ms) / len(post_ms) / 1e3
@@ -459,6 +488,7 @@ def main():
         "frame_latency_ms": round(latency_ms, 4),
         "serial": {"ms_per_frame": round(latency_ms, 4), "mrays_s": round(rays / (latency_ms / 1e3) / 1e6, 4)},
         "pipelined": {"streams": 2, "ms_per_frame": round(ms_per_step, 4), "frames_bitwise_equal_serial": same,
+                      "launch_opts": stream_opts,
                       "speedup_vs_serial": round(latency_ms / ms_per_step, 4)},
         "trace_mrays_s": round(rays / t_trace / 1e6, 4),
         "trace_ms": round(t_trace * 1e3, 4),
