@@ -392,6 +392,16 @@ def main():
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
                  "multi-GPU runs are not part of this build (use --gpus 1)")
+    fixed_opts = None
+    if args.launch_opts:
+        try:
+            fixed_opts = [int(v) for v in args.launch_opts.split(",")]
+        except ValueError:
+            fixed_opts = []
+        if len(fixed_opts) not in (5, 6):
+            sys.exit("--launch-opts takes five or six integers")
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("--steps must be at least 1 and --warmup at least 0")
 
     import torch
 
@@ -407,10 +417,8 @@ def main():
     hdr = r.alloc_hdr()
     bloom = torch.empty_like(hdr)
     ldr = torch.empty((H, W), dtype=torch.int32, device="cuda")
-    if args.launch_opts:
-        vals = [int(v) for v in args.launch_opts.split(",")]
-        if len(vals) not in (5, 6):
-            sys.exit("--launch-opts takes five or six integers")
+    if fixed_opts:
+        vals = fixed_opts
         r.set_launch_opts(*vals)
         launch_opts = dict(zip(("waves_per_cu", "refill_threshold", "queue_chunk", "cost_order", "long_ray_attempts",
                                 "isolate_level"), vals + [0] * (6 - len(vals))))
