@@ -34,10 +34,12 @@ hipError_t launch_trace_fp64(const RenderParams* d_params, int n_renders, uint64
                              DevCounters* d_counters, int blocks, hipStream_t stream);
 hipError_t launch_trace_mixed(const RenderParams* d_params, int n_renders, uint64_t total_items,
                               int refill_threshold, DevCounters* d_counters, int blocks, hipStream_t stream);
-hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, const float* d_wts, float threshold,
-                          int enabled, hipStream_t stream);
-hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int TH,
-                         const float* d_wts, float intensity, int bloom_on, hipStream_t stream);
+hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, const float* d_wts,
+                          const float* d_ws_col, float threshold, hipStream_t stream);
+hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int K, int waves,
+                         const float* d_wts, const float* d_ws_row, float intensity, int bloom_on,
+                         hipStream_t stream);
+bool post_v_plan(int R, size_t max_lds, int* k, int* waves);
 hipError_t launch_cost_order(const RenderParams* d_params, int n_renders, int n_tiles, uint8_t* d_keys,
                              uint32_t* d_hist, uint32_t* d_perm, hipStream_t stream);
 hipError_t launch_trace_fp64_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
@@ -49,7 +51,6 @@ hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& 
                                 uint32_t n_pix, double* part, double* grad_out, hipStream_t stream);
 size_t cost_order_hist_words(int n_tiles);
 size_t bloom_h_lds_bytes(int R);
-size_t post_v_lds_bytes(int R, int TH);
 }  // namespace kt
 
 namespace {
@@ -211,11 +212,13 @@ struct kt_renderer {
     float* d_disk_lut_f = nullptr;
     float* d_bb_lut_f = nullptr;
     float* d_bloom_w = nullptr;
-    int post_th = 64;
-    // Per-launch state in two slots that traces use in turn, so that two
-    // traces issued on two streams run concurrently: the long-ray tail of
-    // one frame then overlaps the bulk of the next.  A slot is reused only
-    // after its previous trace has finished (stream wait on `done`).
+    float* d_bloom_ws = nullptr;  // [W] column then [H] row sums of in-bounds weights
+    int post_k = 8, post_waves = 8;  // post_v tile: rows per lane, waves per block
+    // Per-launch state in kSlots slots that traces use in turn, so that up
+    // to kSlots traces issued on as many streams run concurrently: the
+    // long-ray tail of one frame then overlaps the bulk of the next.  A slot
+    // is reused only after its previous trace has finished (stream wait on
+    // `done`).
     struct Slot {
         kt::DevCounters* counters = nullptr;
         uint32_t* flag = nullptr;       // LUT-overflow flag of kt_trace launches
@@ -228,7 +231,8 @@ struct kt_renderer {
         bool used = false;
         uint64_t rays = 0;
     };
-    Slot slots[2];
+    static constexpr int kSlots = 4;
+    Slot slots[kSlots];
     int next_slot = 0, last_slot = -1;
     size_t max_tiles = 0;
     // ring of parameter records: pinned host staging -> device
@@ -305,6 +309,7 @@ void release(kt_renderer* r) {
     if (r->d_disk_lut_f) (void)hipFree(r->d_disk_lut_f);
     if (r->d_bb_lut_f) (void)hipFree(r->d_bb_lut_f);
     if (r->d_bloom_w) (void)hipFree(r->d_bloom_w);
+    if (r->d_bloom_ws) (void)hipFree(r->d_bloom_ws);
     delete r;
 }
 
@@ -427,7 +432,7 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     sl.flag_used = flag;
     sl.rays = (uint64_t)r->cfg.width * r->cfg.height * r->cfg.spp;
     r->last_slot = r->next_slot;
-    r->next_slot ^= 1;
+    r->next_slot = (r->next_slot + 1) % kt_renderer::kSlots;
     return KT_OK;
 }
 
@@ -436,10 +441,12 @@ int post_into(kt_renderer* r, const float* hdr, float* bloom, uint32_t* ldr, hip
     int W = (int)c.width, H = (int)c.height, R = r->d.bloom_R;
     bool on = r->d.bloom_active;
     if (on) {
-        hipError_t e = kt::launch_bloom_h(hdr, bloom, W, H, R, r->d_bloom_w, (float)c.bloom_threshold, 1, s);
+        hipError_t e = kt::launch_bloom_h(hdr, bloom, W, H, R, r->d_bloom_w, r->d_bloom_ws,
+                                          (float)c.bloom_threshold, s);
         if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("bloom launch: ") + hipGetErrorString(e));
     }
-    hipError_t e = kt::launch_post_v(hdr, bloom, ldr, W, H, R, r->post_th, r->d_bloom_w, (float)c.bloom_intensity,
+    hipError_t e = kt::launch_post_v(hdr, bloom, ldr, W, H, R, r->post_k, r->post_waves, r->d_bloom_w,
+                                     r->d_bloom_ws ? r->d_bloom_ws + W : nullptr, (float)c.bloom_intensity,
                                      on ? 1 : 0, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("post launch: ") + hipGetErrorString(e));
     return KT_OK;
@@ -766,13 +773,10 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
             release(r);
             return set_err(nullptr, KT_ERR_CAPACITY, "bloom radius too large for the LDS-staged blur");
         }
-        int th = 64;
-        while (th > 4 && kt::post_v_lds_bytes(R, th) > r->max_lds) th /= 2;
-        if (kt::post_v_lds_bytes(R, th) > r->max_lds) {
+        if (!kt::post_v_plan(R, r->max_lds, &r->post_k, &r->post_waves)) {
             release(r);
             return set_err(nullptr, KT_ERR_CAPACITY, "bloom radius too large for the LDS-staged blur");
         }
-        r->post_th = th;
     }
     std::vector<float> w((size_t)2 * R + 1, 0.f);
     for (int i = -R; i <= R; ++i) {
@@ -781,6 +785,22 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     }
     KT_CREATE_HIP(hipMalloc(&r->d_bloom_w, sizeof(float) * w.size()), "hipMalloc bloom weights");
     KT_CREATE_HIP(hipMemcpy(r->d_bloom_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice), "upload bloom weights");
+    if (r->d.bloom_active) {
+        // renormalisation: the sum of the in-bounds tap weights of each
+        // column and row, accumulated in tap order (kernel.cu:54-84)
+        const int W = (int)cfg->width, H = (int)cfg->height;
+        std::vector<float> ws((size_t)W + H);
+        auto in_bounds_sum = [&](int p, int n) {
+            float acc = 0.f;
+            for (int i = std::max(-R, -p); i <= std::min(R, n - 1 - p); ++i) acc += w[i + R];
+            return acc;
+        };
+        for (int x = 0; x < W; ++x) ws[x] = in_bounds_sum(x, W);
+        for (int y = 0; y < H; ++y) ws[(size_t)W + y] = in_bounds_sum(y, H);
+        KT_CREATE_HIP(hipMalloc(&r->d_bloom_ws, sizeof(float) * ws.size()), "hipMalloc bloom weight sums");
+        KT_CREATE_HIP(hipMemcpy(r->d_bloom_ws, ws.data(), sizeof(float) * ws.size(), hipMemcpyHostToDevice),
+                      "upload bloom weight sums");
+    }
     r->max_tiles = (size_t)cfg->width * cfg->height;  // tile edge 1 is the largest count
     for (auto& sl : r->slots) {
         KT_CREATE_HIP(hipMalloc(&sl.counters, sizeof(kt::DevCounters)), "hipMalloc counters");
