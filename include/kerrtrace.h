@@ -241,9 +241,9 @@ kt_status kt_finish_frame(kt_renderer* r, int32_t index);
 
 /* ---- stream-ordered device API on caller-owned buffers --------------- */
 /* hdr: float4[W*H] (rgb, 1); hit_class: uint8[W*H] or NULL.
- * Consecutive traces use two alternating sets of launch state (work queue,
- * counters, tile order), so traces issued on two different streams run
- * concurrently on the device; a third waits for the first to finish. */
+ * Consecutive traces use four sets of launch state (work queue, counters,
+ * tile order) in turn, so up to four traces issued on different streams
+ * run concurrently on the device; a fifth waits for the first to finish. */
 kt_status kt_trace(kt_renderer* r, const kt_camera* cam, float* hdr,
                    uint8_t* hit_class, void* stream);
 /* As kt_trace, also writing each pixel's DP5(4) attempt count to
