@@ -7,11 +7,11 @@ One step = one full frame: trace (FP64) + bloom + display transform, with
 every buffer resident in HBM.  Rays are synthetic (pinhole camera), spp 1.
 
 Prints one JSON line.  `value` = rays per second over a run of whole frames
-(Mrays/s) issued as a renderer streams them: consecutive frames alternate
-between two HIP streams with their own HDR, bloom and image buffers, so up to
-two frames are in flight (as in the reference's double-buffered frame loop)
-and one frame's long-ray tail overlaps the next frame's bulk; their launch
-options are tuned for that (`pipelined.launch_opts`).  `serial`
+(Mrays/s) issued as a renderer streams them: consecutive frames take two or
+three HIP streams in turn, each with its own HDR, bloom and image buffers, so
+several frames are in flight (as in the reference's double-buffered frame
+loop) and one frame's long-ray tail overlaps the next frame's bulk; the
+stream count and launch options are tuned for that (`pipelined`).  `serial`
 gives the same frames one at a time (`frame_latency_ms`), `trace_mrays_s`
 the tracer alone.  `roofline` prices the tracer's
 algorithmic FP64 FLOPs (exact per-unit counts from
@@ -93,19 +93,21 @@ def timed_traces(r, cam, hdr, steps):
 
 
 def pipelined_frames(r, cam, bufs, streams, steps):
-    """Frames k = 0..steps-1 on streams[k % 2] into bufs[k % 2] (hdr, bloom,
-    ldr): trace then bloom + display transform, up to two frames in flight.
-    Returns the elapsed ms from the first issue to the end of the last frame
-    (HIP events on the current stream, which waits for both)."""
+    """Frames k = 0..steps-1 on streams[k % n] into bufs[k % n] (hdr, bloom,
+    ldr), n = len(streams): trace then bloom + display transform, up to n
+    frames in flight.  Returns the elapsed ms from the first issue to the end
+    of the last frame (HIP events on the current stream, which waits for all
+    of them)."""
     import torch
     cur = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(cur)
     for s in streams:
         s.wait_event(e0)
+    n = len(streams)
     for k in range(steps):
-        s = streams[k % 2]
-        hdr, bloom, ldr = bufs[k % 2]
+        s = streams[k % n]
+        hdr, bloom, ldr = bufs[k % n]
         r.trace(cam, hdr=hdr, with_class=False, stream=s)
         r.post_process(hdr, ldr=ldr, bloom=bloom, stream=s)
     for s in streams:
@@ -120,22 +122,24 @@ def pipelined_frames(r, cam, bufs, streams, steps):
 STREAM_CANDIDATES = [(w, t, 8, 1, 256, 1) for w in (6, 7, 8) for t in (16, 24, 32)]
 
 
-def tune_streamed(r, cam, bufs, streams, first, frames=6):
-    """Launch options for streamed frames: the frame time of two frames in
-    flight favours other options than a frame alone (more resident waves,
-    rays kept longer per wave).  Tries `first` and STREAM_CANDIDATES, each on
-    `frames` streamed frames, and returns the fastest as a dict."""
+def tune_streamed(r, cam, bufs, streams, first, counts=(2, 3), frames=6):
+    """Launch options and stream count for streamed frames: the frame time
+    of several frames in flight favours other options than a frame alone
+    (more resident waves, rays kept longer per wave).  Tries `first` and
+    STREAM_CANDIDATES on each stream count in `counts`, each on `frames`
+    streamed frames, and returns the fastest (options dict, stream count)."""
     keys = ("waves_per_cu", "refill_threshold", "queue_chunk", "cost_order", "long_ray_attempts", "isolate_level")
     cands = [tuple(first[k] for k in keys)] if first else []
     cands += [c for c in STREAM_CANDIDATES if c not in cands]
-    best, best_ms = None, float("inf")
+    best, best_n, best_ms = None, None, float("inf")
     for c in cands:
         r.set_launch_opts(*c)
-        pipelined_frames(r, cam, bufs, streams, 2)
-        ms = pipelined_frames(r, cam, bufs, streams, frames) / frames
-        if ms < best_ms:
-            best, best_ms = c, ms
-    return dict(zip(keys, best))
+        for n in counts:
+            pipelined_frames(r, cam, bufs[:n], streams[:n], n)
+            ms = pipelined_frames(r, cam, bufs[:n], streams[:n], frames) / frames
+            if ms < best_ms:
+                best, best_n, best_ms = c, n, ms
+    return dict(zip(keys, best)), best_n
 
 
 def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64, launch_opts):
@@ -202,7 +206,8 @@ def uhd_leg(args, launch_opts):
     """3840x2160, i = 60 deg, tol 1e-7 at spins 0, 0.5, 0.9 and 0.998: the
     mixed tracer against the FP64 tracer on the same frame (the FP64 frame
     is the FP64 oracle's bit for bit), each timed with HIP events around
-    its launches, with the 1080p frame's launch options."""
+    its launches, with launch options from the autotuner on each frame
+    (`launch_opts` when given, the defaults with --no-autotune)."""
     import numpy as np
     import torch
 
@@ -218,23 +223,29 @@ def uhd_leg(args, launch_opts):
             c = cfg.copy()
             c.precision = prec
             with HipRenderer(c) as r:
+                h = r.alloc_hdr()
                 if launch_opts:
                     r.set_launch_opts(**launch_opts)
-                h = r.alloc_hdr()
+                    opts = launch_opts
+                elif args.no_autotune:
+                    opts = "default"
+                else:
+                    opts = r.autotune(cam, hdr=h)
                 cls = torch.empty((c.height, c.width), dtype=torch.uint8, device="cuda")
                 r.trace(cam, hdr=h, cls=cls)
                 torch.cuda.synchronize()
                 t, st = timed_traces(r, cam, h, args.steps)
                 r.trace(cam, hdr=h, cls=cls)
                 torch.cuda.synchronize()
-                res[prec] = (t, st, h.cpu().numpy().astype(np.float64), cls.cpu().numpy())
-        t64, st64, h64, c64 = res[K.KT_PRECISION_FP64]
-        tm, stm, hm, cm = res[K.KT_PRECISION_MIXED]
+                res[prec] = (t, st, h.cpu().numpy().astype(np.float64), cls.cpu().numpy(), opts)
+        t64, st64, h64, c64, o64 = res[K.KT_PRECISION_FP64]
+        tm, stm, hm, cm, om = res[K.KT_PRECISION_MIXED]
         m = compare(hm, cm, h64, c64)
         out[f"spin_{spin:g}"] = {
             "fp64_trace_ms": round(t64 * 1e3, 4), "fp64_mrays_s": round(rays / t64 / 1e6, 4),
             "mixed_trace_ms": round(tm * 1e3, 4), "mixed_mrays_s": round(rays / tm / 1e6, 4),
             "speedup": round(t64 / tm, 4),
+            "launch_opts": {"fp64": o64, "mixed": om},
             "attempts_per_ray": {"fp64": round(st64["attempts"] / rays, 3), "mixed": round(stm["attempts"] / rays, 3)},
             "vs_fp64_frame": {k: m[k] for k in ("class_mismatch", "class_mismatch_frac", "class_mismatch_non_boundary",
                                                  "subclass_mismatch", "max_rel", "max_rel_non_boundary", "rms_rel")},
@@ -388,6 +399,9 @@ def main():
                          "refill threshold, tile edge, cost order, long-ray attempts[, isolate level] "
                          "(e.g. 6,32,8,1,256,1)")
     ap.add_argument("--adam-iters", type=int, default=200, help="Adam iterations of the fit (0: none)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="HIP streams of the streamed frames, 1 to 3 (default: tuned between 2 and 3; "
+                         "2 with --launch-opts or --no-autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
@@ -402,6 +416,8 @@ def main():
             sys.exit("--launch-opts takes five or six integers")
     if args.steps < 1 or args.warmup < 0:
         sys.exit("--steps must be at least 1 and --warmup at least 0")
+    if not 0 <= args.streams <= 3:
+        sys.exit("--streams takes 1, 2 or 3 (four streams would share a hardware queue)")
 
     import torch
 
@@ -453,15 +469,22 @@ def main():
     latency_ms = sum(frame_ms) / args.steps
     serial_hdr, serial_ldr = hdr.clone(), ldr.clone()
 
-    # the same frames streamed: two streams, two frames in flight, with
-    # launch options tuned for streaming (unless they were given)
-    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    bufs = [(hdr, bloom, ldr), (r.alloc_hdr(), torch.empty_like(bloom), torch.empty_like(ldr))]
+    # the same frames streamed: two or three streams (a process has four
+    # hardware queues, one taken by the current stream), as many frames in
+    # flight, with the stream count and launch options tuned for streaming
+    # (unless they were given)
+    n_max = max(3, args.streams)
+    streams = [torch.cuda.Stream() for _ in range(n_max)]
+    bufs = [(hdr, bloom, ldr)] + [(r.alloc_hdr(), torch.empty_like(bloom), torch.empty_like(ldr))
+                                  for _ in range(n_max - 1)]
     stream_opts = launch_opts
+    n_streams = args.streams or 2
     if not args.launch_opts and not args.no_autotune:
-        stream_opts = tune_streamed(r, cam, bufs, streams, launch_opts)
+        counts = (args.streams,) if args.streams else (2, 3)
+        stream_opts, n_streams = tune_streamed(r, cam, bufs, streams, launch_opts, counts)
         r.set_launch_opts(**stream_opts)
-    pipelined_frames(r, cam, bufs, streams, 2 * max(1, args.warmup))
+    streams, bufs = streams[:n_streams], bufs[:n_streams]
+    pipelined_frames(r, cam, bufs, streams, n_streams * max(1, args.warmup))
     print("[bench] pipelined warmup done", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     total_ms = pipelined_frames(r, cam, bufs, streams, args.steps)
@@ -491,11 +514,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "1920x1080 a=0.998 i=85deg r_obs=100M fov=20deg tol=1e-9 FP64 trace + bloom(sigma 12) + ACES/sRGB",
-                   "rays": rays, "parallelism": "single GPU, frames streamed on two HIP streams",
+                   "rays": rays, "parallelism": f"single GPU, frames streamed on {n_streams} HIP streams",
                    "launch_opts": launch_opts},
         "frame_latency_ms": round(latency_ms, 4),
         "serial": {"ms_per_frame": round(latency_ms, 4), "mrays_s": round(rays / (latency_ms / 1e3) / 1e6, 4)},
-        "pipelined": {"streams": 2, "ms_per_frame": round(ms_per_step, 4), "frames_bitwise_equal_serial": same,
+        "pipelined": {"streams": n_streams, "ms_per_frame": round(ms_per_step, 4), "frames_bitwise_equal_serial": same,
                       "launch_opts": stream_opts,
                       "speedup_vs_serial": round(latency_ms / ms_per_step, 4)},
         "trace_mrays_s": round(rays / t_trace / 1e6, 4),
@@ -526,7 +549,7 @@ def main():
         out["mixed"] = mixed_leg(cfg, cam, r, hdr, args, fc, t_kernel, launch_opts)
     if not args.no_uhd:
         print("[bench] 4K four-spin leg", file=sys.stderr, flush=True)
-        out["uhd2160_i60"] = uhd_leg(args, launch_opts)
+        out["uhd2160_i60"] = uhd_leg(args, launch_opts if args.launch_opts else None)
     if not args.no_batch:
         print("[bench] batch leg", file=sys.stderr, flush=True)
         out["batch64"] = {"fp64_tol_1e-9": batch_leg(args, K.KT_PRECISION_FP64, 1e-9),

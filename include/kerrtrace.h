@@ -281,10 +281,13 @@ kt_status kt_batch_get_stats(kt_batch* b, kt_trace_stats* out);
  * Traces the frame (kt_autotune: camera cam into the device buffer hdr;
  * kt_batch_autotune: every render into hdr[i]) with each of a fixed set
  * of launch options -- waves per CU, refill threshold (rays a wave keeps
- * before it takes new work), work-order tile edge -- keeps the fastest
- * (best of two runs, HIP events around the trace) as the handle's launch
- * options and returns it in *best (may be NULL).  The buffers hold the
- * last trial's frame afterwards.  Synchronises the stream. */
+ * before it takes new work), work-order tile edge -- keeps the fastest as
+ * the handle's launch options and returns it in *best (may be NULL):
+ * every candidate is timed twice (best run kept, HIP events around the
+ * trace), then the three fastest four more times each, and the lowest
+ * mean of those wins.  On an error the handle keeps its previous options.
+ * The buffers hold the last trial's frame afterwards.  Synchronises the
+ * stream. */
 kt_status kt_autotune(kt_renderer* r, const kt_camera* cam, float* hdr,
                       void* stream, kt_launch_opts* best);
 kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream,

@@ -1416,74 +1416,101 @@ static const kt_launch_opts kTuneGrid[] = {
     {8, 8, 4, 1, 256, 0},  {4, 16, 4, 1, 256, 0}, {6, 32, 8, 1, 256, 1}, {8, 32, 8, 1, 256, 1},
     {4, 32, 8, 1, 256, 1}, {6, 16, 8, 1, 256, 1}, {8, 16, 8, 1, 256, 2}, {6, 32, 8, 1, 256, 2},
 };
-static const int kTuneReps = 2;
+static const int kTuneReps = 2;        // first pass: best of two runs per candidate
+static const int kTuneFinalists = 3;   // second pass: the fastest three ...
+static const int kTuneFinalReps = 4;   // ... timed four more times each, lowest mean wins
+
+}  // extern "C"
+
+// Two-pass choice over kTuneGrid with time_one(opts, &ms) timing one
+// trace.  The best-of-two first pass ranks every candidate cheaply; the
+// mean of the second pass decides among the leaders, so one lucky run
+// cannot pick options that are slower on average.
+template <typename TimeOne>
+int tune_two_pass(TimeOne time_one, kt_launch_opts* best) {
+    const int n = (int)(sizeof kTuneGrid / sizeof kTuneGrid[0]);
+    std::vector<std::pair<float, int>> rank;
+    for (int i = 0; i < n; ++i) {
+        float t = 3.0e38f;
+        for (int k = 0; k < kTuneReps; ++k) {
+            float ms = 0.0f;
+            int rc = time_one(kTuneGrid[i], &ms);
+            if (rc != KT_OK) return rc;
+            t = std::min(t, ms);
+        }
+        rank.emplace_back(t, i);
+    }
+    std::sort(rank.begin(), rank.end());
+    double best_mean = 3.0e38;
+    int best_i = rank[0].second;
+    for (int f = 0; f < std::min(kTuneFinalists, n); ++f) {
+        const int i = rank[f].second;
+        double sum = 0.0;
+        for (int k = 0; k < kTuneFinalReps; ++k) {
+            float ms = 0.0f;
+            int rc = time_one(kTuneGrid[i], &ms);
+            if (rc != KT_OK) return rc;
+            sum += ms;
+        }
+        if (sum / kTuneFinalReps < best_mean) {
+            best_mean = sum / kTuneFinalReps;
+            best_i = i;
+        }
+    }
+    *best = kTuneGrid[best_i];
+    return KT_OK;
+}
+
+extern "C" {
 
 kt_status kt_autotune(kt_renderer* r, const kt_camera* cam, float* hdr, void* stream, kt_launch_opts* best) {
     if (!r || !cam || !hdr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     KT_HIP(r, hipEventCreate(&e0), "event");
-    KT_HIP(r, hipEventCreate(&e1), "event");
-    float best_ms = 3.0e38f;
-    kt_launch_opts best_o = r->opts;
-    int rc = KT_OK;
-    for (const kt_launch_opts& o : kTuneGrid) {
-        r->opts = o;
-        float t_min = 3.0e38f;
-        for (int k = 0; k < kTuneReps && rc == KT_OK; ++k) {
-            hipError_t e = hipEventRecord(e0, s);
-            if (e == hipSuccess) rc = trace_into(r, cam, hdr, nullptr, nullptr, s);
-            if (rc == KT_OK && e == hipSuccess) e = hipEventRecord(e1, s);
-            if (rc == KT_OK && e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.0f;
-            if (rc == KT_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (e != hipSuccess && rc == KT_OK) rc = hip_err(r, e, "autotune timing");
-            if (rc == KT_OK) t_min = std::min(t_min, ms);
-        }
-        if (rc != KT_OK) break;
-        if (t_min < best_ms) {
-            best_ms = t_min;
-            best_o = o;
-        }
+    hipError_t ce = hipEventCreate(&e1);
+    if (ce != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_err(r, ce, "event");
     }
+    const kt_launch_opts before = r->opts;
+    kt_launch_opts best_o = before;
+    int rc = tune_two_pass(
+        [&](const kt_launch_opts& o, float* ms) {
+            r->opts = o;
+            hipError_t e = hipEventRecord(e0, s);
+            if (e != hipSuccess) return hip_err(r, e, "autotune timing");
+            int trc = trace_into(r, cam, hdr, nullptr, nullptr, s);
+            if (trc != KT_OK) return trc;
+            e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+            return e == hipSuccess ? (int)KT_OK : hip_err(r, e, "autotune timing");
+        },
+        &best_o);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    r->opts = best_o;
-    if (best) *best = best_o;
+    r->opts = rc == KT_OK ? best_o : before;
+    if (best && rc == KT_OK) *best = best_o;
     return rc;
 }
 
 kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream, kt_launch_opts* best) {
     if (!b || !hdr) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
-    hipStream_t s = (hipStream_t)stream;
-    float best_ms = 3.0e38f;
-    kt_launch_opts best_o = b->opts;
-    for (const kt_launch_opts& o : kTuneGrid) {
-        b->opts = o;
-        float t_min = 3.0e38f;
-        for (int k = 0; k < kTuneReps; ++k) {
-            int rc = kt_batch_trace(b, hdr, nullptr, stream);
-            if (rc != KT_OK) {
-                b->opts = best_o;
-                return rc;
-            }
+    const kt_launch_opts before = b->opts;
+    kt_launch_opts best_o = before;
+    int rc = tune_two_pass(
+        [&](const kt_launch_opts& o, float* ms) {
+            b->opts = o;
+            int trc = kt_batch_trace(b, hdr, nullptr, stream);
+            if (trc != KT_OK) return trc;
             hipError_t e = hipEventSynchronize(b->ev1);
-            float ms = 0.0f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
-            if (e != hipSuccess) {
-                b->opts = best_o;
-                return hip_err(nullptr, e, "autotune timing");
-            }
-            t_min = std::min(t_min, ms);
-        }
-        if (t_min < best_ms) {
-            best_ms = t_min;
-            best_o = o;
-        }
-    }
-    (void)s;
-    b->opts = best_o;
-    if (best) *best = best_o;
-    return KT_OK;
+            if (e == hipSuccess) e = hipEventElapsedTime(ms, b->ev0, b->ev1);
+            return e == hipSuccess ? (int)KT_OK : hip_err(nullptr, e, "autotune timing");
+        },
+        &best_o);
+    b->opts = rc == KT_OK ? best_o : before;
+    if (best && rc == KT_OK) *best = best_o;
+    return rc;
 }
 }  // extern "C"

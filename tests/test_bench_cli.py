@@ -26,3 +26,9 @@ def test_launch_opts_must_be_five_or_six_integers():
 def test_steps_must_be_positive():
     p = _run("--steps", "0")
     assert p.returncode != 0 and "--steps" in p.stderr
+
+
+def test_streams_must_be_one_to_three():
+    for bad in ("4", "-1"):
+        p = _run("--streams", bad)
+        assert p.returncode != 0 and "--streams" in p.stderr, bad
