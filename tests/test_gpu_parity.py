@@ -142,6 +142,41 @@ def test_post_process_no_bloom_and_threshold(torch_gpu):
         assert d <= 1
 
 
+@pytest.mark.parametrize("w,h,radius", [(211, 37, 1.0), (300, 77, 2.5), (97, 230, 80.0), (120, 150, 90.0),
+                                        (64, 260, 100.0), (70, 240, 104.0), (1, 1, 12.0), (2, 300, 36.0),
+                                        (641, 3, 36.0)])
+def test_post_process_tile_plans(torch_gpu, w, h, radius):
+    """Bloom radii that take every vertical-pass tile plan (8 rows per lane
+    with 8, 4, 2 and 1 waves per block, 1 row per lane for the widest
+    radius), radii whose taps are fewer than a register tile, and frames
+    narrower or shorter than a tile or than the blur: +-1 LSB of the FP64
+    oracle."""
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipRenderer
+    rng = np.random.default_rng(11)
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.width, cfg.height = w, h
+    cfg.bloom_radius = radius
+    say(f"[parity] post {w}x{h} radius {radius}")
+    hh = np.ones((h, w, 4), dtype=np.float32)
+    hh[..., :3] = (rng.random((h, w, 3)) ** 3 * 4.0).astype(np.float32)
+    with HipRenderer(cfg) as r:
+        g_ldr = r.post_process(torch.from_numpy(hh).cuda()).cpu().numpy()
+    o_ldr, _ = O.post(cfg, hh)
+    d, hist = ldr_diff(g_ldr, o_ldr)
+    say(f"[parity]   max diff {d} hist {hist}")
+    assert d <= 1
+
+
+def test_post_process_radius_beyond_lds_is_refused(torch_gpu):
+    from kerrtrace_amd._lib import KerrTraceError
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.bloom_radius = 105.0
+    with pytest.raises(KerrTraceError, match="bloom radius"):
+        HipRenderer(cfg)
+
+
 def test_frame_api_sequencing(torch_gpu):
     """submit/ready/finish semantics of src/renderer/mod.rs:119-260."""
     from kerrtrace_amd._lib import KerrTraceError
