@@ -119,7 +119,8 @@ def pipelined_frames(r, cam, bufs, streams, steps):
     return e0.elapsed_time(e1)
 
 
-STREAM_CANDIDATES = [(w, t, 8, 1, 256, 1) for w in (6, 7, 8) for t in (16, 24, 32)]
+STREAM_CANDIDATES = [(w, t, 8, 1, 256, 1) for w in (6, 7, 8) for t in (16, 24, 32)] + \
+    [(6, 24, 8, 1, 256, 0), (7, 24, 8, 1, 256, 0), (7, 24, 8, 1, 256, 2)]
 
 
 def tune_streamed(r, cam, bufs, streams, first, counts=(2, 3), frames=6):
