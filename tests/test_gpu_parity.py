@@ -219,27 +219,29 @@ def test_frame_api_sequencing(torch_gpu):
         assert ok
 
 
-def test_traces_on_two_streams(torch_gpu):
-    """Consecutive traces alternate between two sets of launch state, so
-    traces issued on two streams run concurrently: every frame of an
-    interleaved run equals the frame traced alone, class buffer included,
+@pytest.mark.parametrize("n_streams", [2, 3, 4])
+def test_traces_on_several_streams(torch_gpu, n_streams):
+    """Consecutive traces take four sets of launch state in turn, so up to
+    four traces issued on different streams run concurrently: every frame of
+    an interleaved run equals the frame traced alone, class buffer included,
     and the counters read afterwards are those of the last trace."""
     torch = torch_gpu
     from kerrtrace_amd.renderer import HipRenderer
     cfg, cam = K.preset("hd1080_a0998")
     cfg, cam = K.scaled(cfg, cam, 240, 136)
-    say("[parity] two streams 240x136")
+    say(f"[parity] {n_streams} streams 240x136")
     with HipRenderer(cfg) as r:
         ref_hdr, ref_cls = r.trace(cam)
         torch.cuda.synchronize()
         ref_st = r.stats()
-        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-        bufs = [(r.alloc_hdr(), torch.empty_like(ref_cls)) for _ in range(3)]
-        for k in range(6):  # three buffers over two streams and two slots
-            hdr, cls = bufs[k % 3]
-            s = streams[k % 2]
-            if k >= 3:
-                s.wait_stream(streams[(k - 3) % 2])  # buffer k % 3 was last written there
+        streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        nb = n_streams + 1
+        bufs = [(r.alloc_hdr(), torch.empty_like(ref_cls)) for _ in range(nb)]
+        for k in range(2 * nb):  # more buffers than streams, more traces than launch-state sets
+            hdr, cls = bufs[k % nb]
+            s = streams[k % n_streams]
+            if k >= nb:
+                s.wait_stream(streams[(k - nb) % n_streams])  # buffer k % nb was last written there
             r.trace(cam, hdr=hdr, cls=cls, stream=s)
         torch.cuda.synchronize()
         st = r.stats()
