@@ -1059,6 +1059,18 @@ int kto_render_rows(const kt_config* cfg, const kt_camera* cam, int row_begin, i
 
 // One DP5(4) attempt on the Kerr right-hand side (for checks of the
 // stepper): out = [k1..k7 (42), ns (6), error (1)].
+// Sky intensity along the unit direction d with transmittance T; *grid = 1
+// on a grid line (cuda/kerr_math.cuh:157-171).
+int kto_sky(const kt_config* cfg, const double d[3], double T, double* intensity, int* grid) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    bool g = false;
+    *intensity = sky_intensity(d[0], d[1], d[2], T, p, &g);
+    *grid = g ? 1 : 0;
+    return KT_OK;
+}
+
 int kto_dp_attempt(const kt_config* cfg, const double s6[6], double pt, double h, double tol, double* out) {
     Params<double> p;
     int rc = make_params(cfg, p);

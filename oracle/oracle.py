@@ -70,10 +70,11 @@ def lib():
         L.kto_post.argtypes = [C.POINTER(KtConfig), C.POINTER(C.c_float), C.POINTER(C.c_uint32), pd]
         L.kto_flop_counts.argtypes = [C.POINTER(KtConfig), pd, d, C.POINTER(C.c_uint64)]
         L.kto_dp_attempt.argtypes = [C.POINTER(KtConfig), pd, d, d, d, pd]
+        L.kto_sky.argtypes = [C.POINTER(KtConfig), pd, d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
-                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir"):
+                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -283,6 +284,15 @@ def dp_attempt(cfg, state6, pt, h, tol):
     out = np.zeros(49, dtype=np.float64)
     _check(lib().kto_dp_attempt(C.byref(cfg), _dptr(s), pt, h, tol, _dptr(out)))
     return out[:42].reshape(7, 6), out[42:48].copy(), float(out[48])
+
+
+def sky(cfg, direction, transmittance=1.0):
+    """Sky (intensity, on a grid line) along a unit direction."""
+    dv = np.ascontiguousarray(direction, dtype=np.float64)
+    out = C.c_double()
+    g = C.c_int()
+    _check(lib().kto_sky(C.byref(cfg), _dptr(dv), transmittance, C.byref(out), C.byref(g)))
+    return out.value, bool(g.value)
 
 
 def trace_dir(cfg, cam_pos, direction, max_log=0):

@@ -562,6 +562,50 @@ def test_noise_and_fbm_match_independent_restatement():
                 assert O.fbm(u, v, octaves, seed) == pytest.approx(expect, abs=1e-12)
 
 
+# ------------------------------------------------------------------ sky
+def _sky_dir(theta, phi):
+    """Unit direction with azimuth theta = atan2(z, x) and latitude
+    phi = asin(y)."""
+    return [math.cos(phi) * math.cos(theta), math.sin(phi), math.cos(phi) * math.sin(theta)]
+
+
+def test_sky_grid_closed_form():
+    """The celestial grid: meridians every pi/divisions in azimuth about the
+    y axis, parallels every pi/divisions in latitude, half-width
+    line_thickness, an equator band three half-widths wide; intensity
+    sky_intensity x transmittance on a line, 0 elsewhere.  Directions are
+    built from (azimuth, latitude), so a wrong axis, spacing, band or
+    scale fails."""
+    cfg = K.default_config()
+    cfg.sky_grid_divisions, cfg.sky_line_thickness, cfg.sky_intensity = 40, 5e-4, 30.0
+    sp, lt = math.pi / 40, 5e-4
+    T = 0.37
+    rng = np.random.default_rng(9)
+    for _ in range(100):
+        i, j = int(rng.integers(-39, 40)), int(rng.integers(2, 19)) * int(rng.choice([-1, 1]))
+        # cell centres: off every line (|latitude| well above the equator band)
+        val, grid = O.sky(cfg, _sky_dir((i + 0.5) * sp, (j + 0.5) * sp), T)
+        assert not grid and val == 0.0
+        # 0.4 half-widths off a meridian, and off a parallel: on a line
+        for th, ph in [(i * sp + 0.4 * lt, (j + 0.5) * sp), ((i + 0.5) * sp, j * sp - 0.4 * lt)]:
+            val, grid = O.sky(cfg, _sky_dir(th, ph), T)
+            assert grid and val == pytest.approx(30.0 * T, rel=1e-15)
+        # 1.6 half-widths off both: in the cell
+        val, grid = O.sky(cfg, _sky_dir(i * sp + 1.6 * lt, j * sp + 1.6 * lt), T)
+        assert not grid
+    # the equator band: three half-widths on either side of latitude 0
+    assert O.sky(cfg, _sky_dir(0.5 * sp, 2.5 * lt), T)[1]
+    assert O.sky(cfg, _sky_dir(0.5 * sp, -2.5 * lt), T)[1]
+    assert not O.sky(cfg, _sky_dir(0.5 * sp, 3.5 * lt), T)[1]
+    # the azimuth is atan2(z, x): with 7 divisions a quarter turn is not a
+    # multiple of the spacing, so the mid-cell azimuth 0.5 sp (on a meridian
+    # if x and z were swapped: pi/2 - 0.5 sp = 3 sp) stays off the grid
+    cfg.sky_grid_divisions = 7
+    sp7 = math.pi / 7
+    assert not O.sky(cfg, _sky_dir(0.5 * sp7, 0.5 * sp7), T)[1]
+    assert O.sky(cfg, _sky_dir(3.0 * sp7, 0.5 * sp7), T)[1]
+
+
 # ---------------------------------------------------- post processing
 def _brute_bloom(h3, R, sigma):
     from scipy.ndimage import correlate1d
