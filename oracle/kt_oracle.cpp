@@ -613,15 +613,14 @@ struct RayOut {
     double overflow_T;
 };
 
-// cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
+// The ray's initial state: the unit pixel direction n (kerr_math.cuh:21-25)
+// and the null momentum at the camera (kerr_trace.cuh:21-44), the covector
+// g k of k = (kt, n).  Also the camera position as the state's position.
 template <class S>
-void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
-               StepLog* log = nullptr, const double* replay_h = nullptr, int replay_n = 0) {
-    // normalize (kerr_math.cuh:21-25)
+void init_ray(const S cam[3], const S dir[3], const Params<S>& p, State<S>& s, S& pt, S& kt) {
     S dd = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
     S inv_n = 1.0 / sqrt_(dd);
     S nx = dir[0] * inv_n, ny = dir[1] * inv_n, nz = dir[2] * inv_n;
-    State<S> s;
     s.x = cam[0]; s.y = cam[1]; s.z = cam[2];
     // initial null momentum (kerr_trace.cuh:21-44)
     S a2 = p.aa;
@@ -642,12 +641,21 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
     S g_tn = 2.0 * H * ldotn;
     S g_nn = 1.0 + 2.0 * H * ldotn * ldotn;
     S disc = g_tn * g_tn - g_tt * g_nn;
-    S kt = (-g_tn - sqrt_(disc)) / g_tt;
-    S pt = g_tt * kt + g_tn;
+    kt = (-g_tn - sqrt_(disc)) / g_tt;
+    pt = g_tt * kt + g_tn;
     S scale = kt + ldotn;
     s.px = nx + 2.0 * H * lx * scale;
     s.py = ny + 2.0 * H * ly * scale;
     s.pz = nz + 2.0 * H * lz * scale;
+}
+
+// cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
+template <class S>
+void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
+               StepLog* log = nullptr, const double* replay_h = nullptr, int replay_n = 0) {
+    State<S> s;
+    S pt, kt;
+    init_ray(cam, dir, p, s, pt, kt);
 
     S col[3] = {S(0.0), S(0.0), S(0.0)};
     S transmittance(1.0);
@@ -949,36 +957,18 @@ int kto_get_derivs(const kt_config* cfg, const double s6[6], double pt, double o
     return KT_OK;
 }
 
-// Initial state of one ray: s6 (x,y,z,px,py,pz), pt, kt.  Repeats the
-// initialisation of trace_ray so tests can check the null condition.
+// Initial state of one ray: s6 (x,y,z,px,py,pz), pt, kt -- trace_ray's
+// own initialisation, so tests can check its null condition and direction.
 int kto_init_ray(const kt_config* cfg, const double cam[3], const double dir[3], double s6[6], double* pt_out,
                  double* kt_out) {
     Params<double> p;
     int rc = make_params(cfg, p);
     if (rc != KT_OK) return rc;
-    double dd = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
-    double inv_n = 1.0 / std::sqrt(dd);
-    double nx = dir[0] * inv_n, ny = dir[1] * inv_n, nz = dir[2] * inv_n;
-    double x = cam[0], y = cam[1], z = cam[2];
-    double a2 = p.aa;
-    double rho2 = std::fma(x, x, std::fma(y, y, z * z));
-    double u = rho2 - a2;
-    double s_term = std::sqrt(std::fma(u, u, 4.0 * a2 * y * y));
-    double r2 = 0.5 * (u + s_term);
-    double r = std::sqrt(r2);
-    double inv_denom = 1.0 / (r2 + a2);
-    double lx = (r * x - p.a * z) * inv_denom, ly = y / r, lz = (r * z + p.a * x) * inv_denom;
-    double H = p.M * r2 * r * (1.0 / std::fma(r2, r2, a2 * y * y));
-    double ldotn = std::fma(lx, nx, std::fma(ly, ny, lz * nz));
-    double g_tt = -1.0 + 2.0 * H, g_tn = 2.0 * H * ldotn, g_nn = 1.0 + 2.0 * H * ldotn * ldotn;
-    double disc = g_tn * g_tn - g_tt * g_nn;
-    double kt = (-g_tn - std::sqrt(disc)) / g_tt;
-    double pt = g_tt * kt + g_tn;
-    double sc = kt + ldotn;
-    s6[0] = x; s6[1] = y; s6[2] = z;
-    s6[3] = nx + 2.0 * H * lx * sc;
-    s6[4] = ny + 2.0 * H * ly * sc;
-    s6[5] = nz + 2.0 * H * lz * sc;
+    State<double> s;
+    double pt = 0.0, kt = 0.0;
+    init_ray(cam, dir, p, s, pt, kt);
+    s6[0] = s.x; s6[1] = s.y; s6[2] = s.z;
+    s6[3] = s.px; s6[4] = s.py; s6[5] = s.pz;
     *pt_out = pt;
     *kt_out = kt;
     return KT_OK;

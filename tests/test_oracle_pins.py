@@ -192,6 +192,9 @@ def test_rhs_is_hamiltonian_gradient(a):
 
 
 def test_initial_momentum_is_null():
+    """The camera ray's momentum is the covector g k of the null vector
+    k = (kt, n): the Hamiltonian vanishes, and the coordinate velocity
+    dx/dlambda it produces is the unit pixel direction n itself."""
     cfg, cam = K.preset("hd1080_a0998")
     rng = np.random.default_rng(4)
     for _ in range(50):
@@ -200,6 +203,9 @@ def test_initial_momentum_is_null():
         Hv = _hamiltonian_mp([mp.mpf(float(v)) for v in s], mp.mpf(pt), cfg.spin, 1)
         assert abs(float(Hv)) < 1e-13
         assert pt < 0  # future-directed: E = -p_t > 0
+        n = d / np.linalg.norm(d)
+        assert np.allclose(O.get_derivs(cfg, s, pt)[:3], n, rtol=0, atol=1e-13)
+        assert kt > 0
 
 
 # ------------------------------------------------------- DP5(4) stepper
