@@ -649,6 +649,20 @@ void init_ray(const S cam[3], const S dir[3], const Params<S>& p, State<S>& s, S
     s.pz = nz + 2.0 * H * lz * scale;
 }
 
+// A step's error tolerance scaled by the Kerr-Schild potential H at its
+// start state (cuda/kerr_trace.cuh:67-73): tol (5 - 5 t), t = (H - 0.05) /
+// (0.3 - 0.05) clamped to [0, 1], with the D1 floor: tol max(5 - 5 t, floor).
+template <class S>
+double step_tol(const State<S>& s, const Params<S>& p) {
+    S r_curr = ks_r(s.x, s.y, s.z, p.aa);
+    S r2_curr = r_curr * r_curr;
+    S dH = fma_(r2_curr, r2_curr, p.aa * s.y * s.y);
+    S Hc = p.M * r_curr * r2_curr * (1.0 / dH);
+    S tq = (Hc - 0.05) / (0.3 - 0.05);
+    tq = fmin_(fmax_(tq, S(0.0)), S(1.0));
+    return p.tol * std::fmax(val(5.0 - 5.0 * tq), p.tol_floor);
+}
+
 // cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
 template <class S>
 void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
@@ -670,14 +684,7 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
     const int step_limit = replay_h ? std::min(p.max_steps, replay_n) : p.max_steps;
     for (int i = 0; i < step_limit && transmittance > p.cutoff; ++i) {
         State<S> k1, k2, k3, k4, k5, k6, k7, ns;
-        // tolerance scaled by the potential (kerr_trace.cuh:67-73), D1 floor
-        S r_curr = ks_r(s.x, s.y, s.z, p.aa);
-        S r2_curr = r_curr * r_curr;
-        S dH = fma_(r2_curr, r2_curr, p.aa * s.y * s.y);
-        S Hc = p.M * r_curr * r2_curr * (1.0 / dH);
-        S tq = (Hc - 0.05) / (0.3 - 0.05);
-        tq = fmin_(fmax_(tq, S(0.0)), S(1.0));
-        double tol = p.tol * std::fmax(val(5.0 - 5.0 * tq), p.tol_floor);
+        const double tol = step_tol(s, p);
         int attempts = 0;
         bool accepted = false;
         S h_used = h;
@@ -1058,6 +1065,15 @@ int kto_sky(const kt_config* cfg, const double d[3], double T, double* intensity
     bool g = false;
     *intensity = sky_intensity(d[0], d[1], d[2], T, p, &g);
     *grid = g ? 1 : 0;
+    return KT_OK;
+}
+
+// The tolerance of a step starting at s6 (x, y, z, px, py, pz).
+int kto_step_tol(const kt_config* cfg, const double s6[6], double* tol) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    *tol = step_tol(State<double>{s6[0], s6[1], s6[2], s6[3], s6[4], s6[5]}, p);
     return KT_OK;
 }
 

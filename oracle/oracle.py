@@ -71,10 +71,12 @@ def lib():
         L.kto_flop_counts.argtypes = [C.POINTER(KtConfig), pd, d, C.POINTER(C.c_uint64)]
         L.kto_dp_attempt.argtypes = [C.POINTER(KtConfig), pd, d, d, d, pd]
         L.kto_sky.argtypes = [C.POINTER(KtConfig), pd, d, pd, C.POINTER(C.c_int)]
+        L.kto_step_tol.argtypes = [C.POINTER(KtConfig), pd, pd]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
-                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky"):
+                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
+                     "kto_step_tol"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -284,6 +286,14 @@ def dp_attempt(cfg, state6, pt, h, tol):
     out = np.zeros(49, dtype=np.float64)
     _check(lib().kto_dp_attempt(C.byref(cfg), _dptr(s), pt, h, tol, _dptr(out)))
     return out[:42].reshape(7, 6), out[42:48].copy(), float(out[48])
+
+
+def step_tol(cfg, state6):
+    """Error tolerance of a step that starts at state6."""
+    sv = np.ascontiguousarray(state6, dtype=np.float64)
+    out = C.c_double()
+    _check(lib().kto_step_tol(C.byref(cfg), _dptr(sv), C.byref(out)))
+    return out.value
 
 
 def sky(cfg, direction, transmittance=1.0):

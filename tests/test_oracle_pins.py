@@ -568,6 +568,30 @@ def test_noise_and_fbm_match_independent_restatement():
                 assert O.fbm(u, v, octaves, seed) == pytest.approx(expect, abs=1e-12)
 
 
+# ---------------------------------------------------- step tolerance
+@pytest.mark.parametrize("floor", [0.0, 1.0])
+def test_step_tolerance_closed_form(floor):
+    """tol(H) = tol max(5 - 5 t, floor), t = clamp((H - 0.05) / 0.25, 0, 1)
+    with the Kerr-Schild potential H = M r^3 / (r^4 + a^2 y^2) at the step's
+    start (mpmath), from far (t = 0) through the ramp to near the hole (t = 1)."""
+    cfg, _ = K.preset("hd1080_a0998")
+    cfg.tol_floor = floor
+    a, M, tau = cfg.spin, cfg.mass, cfg.tolerance
+    rng = np.random.default_rng(13)
+    seen = set()
+    for radius in np.concatenate([rng.uniform(1.6, 40.0, 150), [2.0, 3.0, 3.9, 4.2, 20.0, 100.0]]):
+        d = rng.normal(size=3)
+        pos = radius * d / np.linalg.norm(d)
+        r = _ks_r_mp(*pos, a)
+        H = M * r**3 / (r**4 + mp.mpf(a) ** 2 * mp.mpf(pos[1]) ** 2)
+        t = min(max((H - mp.mpf("0.05")) / mp.mpf("0.25"), 0), 1)
+        expect = tau * max(float(5 - 5 * t), floor)
+        got = O.step_tol(cfg, list(pos) + [0.3, -0.2, 0.9])
+        assert got == pytest.approx(expect, rel=1e-12, abs=1e-30)
+        seen.add("far" if t == 0 else "near" if t == 1 else "ramp")
+    assert seen == {"far", "near", "ramp"}
+
+
 # ------------------------------------------------------------------ sky
 def _sky_dir(theta, phi):
     """Unit direction with azimuth theta = atan2(z, x) and latitude
