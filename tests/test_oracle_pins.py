@@ -503,6 +503,40 @@ def test_blackbody_lut_physics():
     assert np.all(np.abs(c - 1) < 0.12), c
 
 
+def _ybar_wyman(lam):
+    """CIE 1931 y-bar, the multi-lobe fit of Wyman, Sloan & Shirley (JCGT
+    2013, eq. 2)."""
+    def g(x, mu, s1, s2):
+        return math.exp(-0.5 * ((x - mu) / (s1 if x < mu else s2)) ** 2)
+    return 0.821 * g(lam, 568.8, 46.9, 40.5) + 0.286 * g(lam, 530.9, 16.3, 31.1)
+
+
+def test_blackbody_lut_luminance_is_planck_integral():
+    """The LUT's luminance Y (back through the IEC 61966-2-1 matrix) against
+    the continuous integral of Planck's law (CODATA second radiation constant
+    c2 = 1.438777e7 nm K, radiance 1e15 / (lam^5 (exp(c2 / lam T) - 1))) times
+    y-bar over [380, 720] nm (scipy quad): the reference's 10 nm sum with both
+    end points agrees to 1e-3 (1e-4 seen), and ratios between temperatures
+    to 5e-4 -- a wrong step factor, power of lambda or c2 would not."""
+    from scipy.integrate import quad
+    cfg, _ = K.preset("cpu256_a09")
+    _, _, bb = O.derive(cfg)
+    n = cfg.lut_size
+    c2 = 1.438777e7
+    Ys, Is = [], []
+    for T in (2000.0, 3000.0, 4500.0, 6500.0, 10000.0, 20000.0):
+        i = int(round(T / cfg.lut_max_temp * (n - 1)))
+        Ti = i / (n - 1) * cfg.lut_max_temp
+        Y = float((SRGB_TO_XYZ @ bb[i])[1])
+        I = quad(lambda lam: 1e15 / (lam**5 * math.expm1(c2 / (lam * Ti))) * _ybar_wyman(lam), 380.0, 720.0,
+                 limit=200)[0]
+        assert Y == pytest.approx(I, rel=1e-3), (T, Y, I)
+        Ys.append(Y)
+        Is.append(I)
+    for k in range(1, len(Ys)):
+        assert Ys[k] / Ys[0] == pytest.approx(Is[k] / Is[0], rel=5e-4)
+
+
 # ------------------------------------------------------- noise
 def test_noise_lattice_zero_and_fbm_flat():
     for ix, iy in [(0, 0), (3, -7), (-12, 40)]:
