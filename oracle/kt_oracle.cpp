@@ -663,6 +663,36 @@ double step_tol(const State<S>& s, const Params<S>& p) {
     return p.tol * std::fmax(val(5.0 - 5.0 * tq), p.tol_floor);
 }
 
+// Redshift factor g of a disk hit at radius r_hit with momentum p_hit
+// (cuda/kerr_trace.cuh:177-187): the Keplerian emitter's u^t =
+// 1/sqrt(1 - 3M/r + 2a sqrt(M)/r^1.5) and angular velocity
+// -sqrt(M)/(r^1.5 + a sqrt(M)) (spin along +y), and the photon's L/E with
+// the reference's frame term 2H (l.xi)(l.p + kt).  orbit = false (g = 0)
+// where no circular orbit exists.
+template <class S>
+S disk_redshift(S x_hit, S z_hit, S px_hit, S py_hit, S pz_hit, S pt, S kt, S r_hit, const Params<S>& p,
+                bool& orbit) {
+    S sqrt_M = sqrt_(p.M), r_sqrt = sqrt_(r_hit);
+    S disc_denom = 1.0 - 3.0 * p.M / r_hit + 2.0 * p.a * sqrt_M / (r_hit * r_sqrt);
+    orbit = disc_denom > 0.0;
+    if (!orbit) return S(0.0);
+    S omega = -sqrt_M / (r_hit * r_sqrt + p.a * sqrt_M);
+    S r2_hit = r_hit * r_hit;
+    S denom_hit = r2_hit + p.aa;
+    S inv_dh = 1.0 / denom_hit;
+    S hlx = (r_hit * x_hit - p.a * z_hit) * inv_dh;
+    S hly(0.0);
+    S hlz = (r_hit * z_hit + p.a * x_hit) * inv_dh;
+    S ldotp = fma_(hlx, px_hit, fma_(hly, py_hit, hlz * pz_hit));
+    S ldotxi = hlx * z_hit - hlz * x_hit;
+    S dHh = r2_hit * r2_hit;
+    S Hh = p.M * r_hit * r2_hit * (1.0 / dHh);
+    S L = (z_hit * px_hit - x_hit * pz_hit) + 2.0 * Hh * ldotxi * (ldotp + kt);
+    S energy = -pt;
+    S l_over_e = L / energy;
+    return 1.0 / ((1.0 / sqrt_(disc_denom)) * (1.0 - omega * l_over_e));
+}
+
 // cuda/kerr_trace.cuh:4-240 -- one ray.  cam/dir may carry tangents.
 template <class S>
 void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& out, RayStats& st,
@@ -750,24 +780,9 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
             S r_hit = ks_r(x_hit, S(0.0), z_hit, p.aa);
             if (r_hit >= p.disk_inner && r_hit <= p.disk_outer) {
                 // Keplerian disk emission (kerr_trace.cuh:177-199)
-                S sqrt_M = sqrt_(p.M), r_sqrt = sqrt_(r_hit);
-                S disc_denom = 1.0 - 3.0 * p.M / r_hit + 2.0 * p.a * sqrt_M / (r_hit * r_sqrt);
-                if (disc_denom > 0.0) {
-                    S omega = -sqrt_M / (r_hit * r_sqrt + p.a * sqrt_M);
-                    S r2_hit = r_hit * r_hit;
-                    S denom_hit = r2_hit + p.aa;
-                    S inv_dh = 1.0 / denom_hit;
-                    S hlx = (r_hit * x_hit - p.a * z_hit) * inv_dh;
-                    S hly(0.0);
-                    S hlz = (r_hit * z_hit + p.a * x_hit) * inv_dh;
-                    S ldotp = fma_(hlx, px_hit, fma_(hly, py_hit, hlz * pz_hit));
-                    S ldotxi = hlx * z_hit - hlz * x_hit;
-                    S dHh = r2_hit * r2_hit;
-                    S Hh = p.M * r_hit * r2_hit * (1.0 / dHh);
-                    S L = (z_hit * px_hit - x_hit * pz_hit) + 2.0 * Hh * ldotxi * (ldotp + kt);
-                    S energy = -pt;
-                    S l_over_e = L / energy;
-                    S g = 1.0 / ((1.0 / sqrt_(disc_denom)) * (1.0 - omega * l_over_e));
+                bool orbit = false;
+                S g = disk_redshift(x_hit, z_hit, px_hit, py_hit, pz_hit, pt, kt, r_hit, p, orbit);
+                if (orbit) {
                     S dspan = p.disk_outer - p.disk_inner;
                     S uu = dspan > 0.0 ? (r_hit - p.disk_inner) / dspan : S(0.0);
                     S sampled = fetch_disk(uu, p);
@@ -1065,6 +1080,18 @@ int kto_sky(const kt_config* cfg, const double d[3], double T, double* intensity
     bool g = false;
     *intensity = sky_intensity(d[0], d[1], d[2], T, p, &g);
     *grid = g ? 1 : 0;
+    return KT_OK;
+}
+
+// Redshift factor of a disk hit: h6 = (x, z, px, py, pz, r); *orbit = 0
+// where no circular orbit exists.
+int kto_disk_redshift(const kt_config* cfg, const double h6[6], double pt, double kt, double* g, int* orbit) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    bool o = false;
+    *g = disk_redshift(h6[0], h6[1], h6[2], h6[3], h6[4], pt, kt, h6[5], p, o);
+    *orbit = o ? 1 : 0;
     return KT_OK;
 }
 

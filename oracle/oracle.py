@@ -72,11 +72,12 @@ def lib():
         L.kto_dp_attempt.argtypes = [C.POINTER(KtConfig), pd, d, d, d, pd]
         L.kto_sky.argtypes = [C.POINTER(KtConfig), pd, d, pd, C.POINTER(C.c_int)]
         L.kto_step_tol.argtypes = [C.POINTER(KtConfig), pd, pd]
+        L.kto_disk_redshift.argtypes = [C.POINTER(KtConfig), pd, d, d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
                      "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
-                     "kto_step_tol"):
+                     "kto_step_tol", "kto_disk_redshift"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -286,6 +287,15 @@ def dp_attempt(cfg, state6, pt, h, tol):
     out = np.zeros(49, dtype=np.float64)
     _check(lib().kto_dp_attempt(C.byref(cfg), _dptr(s), pt, h, tol, _dptr(out)))
     return out[:42].reshape(7, 6), out[42:48].copy(), float(out[48])
+
+
+def disk_redshift(cfg, x, z, px, py, pz, r, pt, kt):
+    """(g, circular orbit exists) of a disk hit."""
+    h = np.array([x, z, px, py, pz, r], dtype=np.float64)
+    g = C.c_double()
+    o = C.c_int()
+    _check(lib().kto_disk_redshift(C.byref(cfg), _dptr(h), pt, kt, C.byref(g), C.byref(o)))
+    return g.value, bool(o.value)
 
 
 def step_tol(cfg, state6):

@@ -626,6 +626,40 @@ def test_step_tolerance_closed_form(floor):
     assert seen == {"far", "near", "ramp"}
 
 
+# ---------------------------------------------------- disk redshift
+def test_disk_redshift_schwarzschild_closed_form():
+    """a = 0: g = sqrt(1 - 3M/r) / (1 - Omega L/E) for a Keplerian emitter
+    (u^t = (1 - 3M/r)^-1/2, Omega = -sqrt(M/r^3) about +y in these axes) and
+    a photon of axial angular momentum L = z p_x - x p_z, energy E = -p_t;
+    the frame term of the reference's L vanishes (l . xi = 0 at a = 0)."""
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.spin, cfg.mass = 0.0, 1.0
+    rng = np.random.default_rng(17)
+    for _ in range(200):
+        r = rng.uniform(3.2, 40.0)
+        phi = rng.uniform(0, 2 * math.pi)
+        x, z = r * math.cos(phi), r * math.sin(phi)
+        px, py, pz = rng.normal(size=3)
+        E = rng.uniform(0.5, 2.0)
+        kt = rng.normal()
+        g, orbit = O.disk_redshift(cfg, x, z, px, py, pz, r, -E, kt)
+        L = z * px - x * pz
+        expect = math.sqrt(1 - 3 / r) / (1 + L / (E * r**1.5))
+        assert orbit and g == pytest.approx(expect, rel=1e-13)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9, 0.998])
+def test_disk_circular_orbits_end_at_the_photon_orbit(a):
+    """Circular orbits (and so disk emission) exist exactly outside the
+    prograde equatorial photon orbit r_ph = 2M (1 + cos(2/3 acos(-a)))."""
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.spin, cfg.mass = a, 1.0
+    r_ph = 2.0 * (1.0 + math.cos(2.0 / 3.0 * math.acos(-a)))
+    for r, inside in [(r_ph * (1 - 1e-9), True), (r_ph * (1 + 1e-9), False)]:
+        _, orbit = O.disk_redshift(cfg, r, 0.0, 0.1, 0.0, 0.2, r, -1.0, 0.5)
+        assert orbit != inside, (a, r, r_ph)
+
+
 # ------------------------------------------------------------------ sky
 def _sky_dir(theta, phi):
     """Unit direction with azimuth theta = atan2(z, x) and latitude
