@@ -1095,6 +1095,25 @@ int kto_disk_redshift(const kt_config* cfg, const double h6[6], double pt, doubl
     return KT_OK;
 }
 
+// The lookup-table fetches of the shading: disk factor at u, blackbody rgb
+// at temperature T (*overflow = 1 above lut_max_temp).
+int kto_fetch_disk(const kt_config* cfg, double u, double* out) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    *out = fetch_disk(u, p);
+    return KT_OK;
+}
+int kto_fetch_blackbody(const kt_config* cfg, double T, double rgb[3], int* overflow) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    bool ovf = false;
+    fetch_blackbody(T, p, rgb, &ovf);
+    *overflow = ovf ? 1 : 0;
+    return KT_OK;
+}
+
 // The tolerance of a step starting at s6 (x, y, z, px, py, pz).
 int kto_step_tol(const kt_config* cfg, const double s6[6], double* tol) {
     Params<double> p;

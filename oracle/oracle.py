@@ -73,11 +73,13 @@ def lib():
         L.kto_sky.argtypes = [C.POINTER(KtConfig), pd, d, pd, C.POINTER(C.c_int)]
         L.kto_step_tol.argtypes = [C.POINTER(KtConfig), pd, pd]
         L.kto_disk_redshift.argtypes = [C.POINTER(KtConfig), pd, d, d, pd, C.POINTER(C.c_int)]
+        L.kto_fetch_disk.argtypes = [C.POINTER(KtConfig), d, pd]
+        L.kto_fetch_blackbody.argtypes = [C.POINTER(KtConfig), d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
                      "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
-                     "kto_step_tol", "kto_disk_redshift"):
+                     "kto_step_tol", "kto_disk_redshift", "kto_fetch_disk", "kto_fetch_blackbody"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -296,6 +298,21 @@ def disk_redshift(cfg, x, z, px, py, pz, r, pt, kt):
     o = C.c_int()
     _check(lib().kto_disk_redshift(C.byref(cfg), _dptr(h), pt, kt, C.byref(g), C.byref(o)))
     return g.value, bool(o.value)
+
+
+def fetch_disk(cfg, u):
+    """The disk factor lookup at u in [0, 1]."""
+    out = C.c_double()
+    _check(lib().kto_fetch_disk(C.byref(cfg), u, C.byref(out)))
+    return out.value
+
+
+def fetch_blackbody(cfg, T):
+    """(rgb, overflow) of the blackbody lookup at temperature T."""
+    rgb = np.zeros(3, dtype=np.float64)
+    o = C.c_int()
+    _check(lib().kto_fetch_blackbody(C.byref(cfg), T, _dptr(rgb), C.byref(o)))
+    return rgb, bool(o.value)
 
 
 def step_tol(cfg, state6):

@@ -626,6 +626,52 @@ def test_step_tolerance_closed_form(floor):
     assert seen == {"far", "near", "ramp"}
 
 
+# ------------------------------------------------------ LUT fetches
+def _tex1d_linear_clamp(table, x):
+    """A 1-D texture fetch at normalised coordinate x with linear filtering
+    and clamp addressing (texel centres at (i + 0.5) / N): the texel
+    coordinate x N - 0.5 split into floor and fraction, neighbours clamped
+    to [0, N - 1], blended linearly (exact arithmetic, no 8-bit weight)."""
+    n = len(table)
+    xb = x * n - 0.5
+    i = math.floor(xb)
+    w = xb - i
+    t0 = table[min(max(i, 0), n - 1)]
+    t1 = table[min(max(i + 1, 0), n - 1)]
+    return t0 + w * (t1 - t0)
+
+
+def test_lut_fetches_are_texture_filtering():
+    """The disk factor at u = (r - r_in)/(r_out - r_in) and the blackbody
+    colour at u = (T/T_max (N - 1) + 0.5)/N, as the reference's tex1D calls
+    address them (cuda/kerr_trace.cuh:196-198, cuda/kernel.cu:1-16), against
+    the linear/clamp filtering definition: below the first texel centre and
+    above the last the end entries, on texel centres the entries themselves;
+    above T_max the fetch reports an overflow and no colour."""
+    cfg, _ = K.preset("cpu256_a09")
+    _, disk, bb = O.derive(cfg)
+    n = cfg.lut_size
+    rng = np.random.default_rng(19)
+    us = list(rng.uniform(0, 1, 300)) + [0.0, 0.1 / n, 0.5 / n, 1.0 - 0.5 / n, 1.0 - 0.1 / n, 1.0,
+                                          (7 + 0.5) / n, (n - 3 + 0.5) / n]
+    for u in us:
+        assert O.fetch_disk(cfg, u) == pytest.approx(_tex1d_linear_clamp(disk, u), rel=1e-12, abs=1e-15)
+    assert O.fetch_disk(cfg, 0.0) == disk[0] and O.fetch_disk(cfg, 1.0) == disk[-1]
+    bbr = bb.reshape(-1, 3)
+    Ts = list(rng.uniform(0, cfg.lut_max_temp, 300)) + [0.0, cfg.lut_max_temp, 1234.5]
+    for T in Ts:
+        rgb, ovf = O.fetch_blackbody(cfg, T)
+        assert not ovf
+        u = (T / cfg.lut_max_temp * (n - 1) + 0.5) / n
+        for c in range(3):
+            want = _tex1d_linear_clamp(bbr[:, c], u)
+            assert rgb[c] == pytest.approx(want, rel=1e-9, abs=1e-12 * abs(bbr[:, c]).max())
+    rgb, ovf = O.fetch_blackbody(cfg, cfg.lut_max_temp)
+    assert np.array_equal(rgb, bbr[-1])
+    rgb, ovf = O.fetch_blackbody(cfg, cfg.lut_max_temp * 1.0001)
+    assert ovf and np.all(rgb == 0.0)
+
+
 # ---------------------------------------------------- disk redshift
 def test_disk_redshift_schwarzschild_closed_form():
     """a = 0: g = sqrt(1 - 3M/r) / (1 - Omega L/E) for a Keplerian emitter
