@@ -1095,6 +1095,15 @@ int kto_disk_redshift(const kt_config* cfg, const double h6[6], double pt, doubl
     return KT_OK;
 }
 
+// Boyer-Lindquist (r, sin theta, cos theta, phi) of a Kerr-Schild point.
+int kto_ks_bl(const kt_config* cfg, const double xyz[3], double out[4]) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    ks_bl(xyz[0], xyz[1], xyz[2], p, out[0], out[1], out[2], out[3]);
+    return KT_OK;
+}
+
 // The lookup-table fetches of the shading: disk factor at u, blackbody rgb
 // at temperature T (*overflow = 1 above lut_max_temp).
 int kto_fetch_disk(const kt_config* cfg, double u, double* out) {

@@ -74,12 +74,14 @@ def lib():
         L.kto_step_tol.argtypes = [C.POINTER(KtConfig), pd, pd]
         L.kto_disk_redshift.argtypes = [C.POINTER(KtConfig), pd, d, d, pd, C.POINTER(C.c_int)]
         L.kto_fetch_disk.argtypes = [C.POINTER(KtConfig), d, pd]
+        L.kto_ks_bl.argtypes = [C.POINTER(KtConfig), pd, pd]
         L.kto_fetch_blackbody.argtypes = [C.POINTER(KtConfig), d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
                      "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
-                     "kto_step_tol", "kto_disk_redshift", "kto_fetch_disk", "kto_fetch_blackbody"):
+                     "kto_step_tol", "kto_disk_redshift", "kto_fetch_disk", "kto_fetch_blackbody",
+                     "kto_ks_bl"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -298,6 +300,14 @@ def disk_redshift(cfg, x, z, px, py, pz, r, pt, kt):
     o = C.c_int()
     _check(lib().kto_disk_redshift(C.byref(cfg), _dptr(h), pt, kt, C.byref(g), C.byref(o)))
     return g.value, bool(o.value)
+
+
+def ks_bl(cfg, x, y, z):
+    """Boyer-Lindquist (r, sin theta, cos theta, phi) of a Kerr-Schild point."""
+    p = np.array([x, y, z], dtype=np.float64)
+    out = np.zeros(4, dtype=np.float64)
+    _check(lib().kto_ks_bl(C.byref(cfg), _dptr(p), _dptr(out)))
+    return tuple(float(v) for v in out)
 
 
 def fetch_disk(cfg, u):

@@ -626,6 +626,29 @@ def test_step_tolerance_closed_form(floor):
     assert seen == {"far", "near", "ramp"}
 
 
+@pytest.mark.parametrize("a", [0.0, 0.5, -0.9, 0.998])
+def test_ks_to_bl_inverts_the_coordinate_map(a):
+    """Kerr-Schild Cartesian coordinates of a Boyer-Lindquist point (spin
+    along +y): y = r cos(theta), z = (r cos(phi) - a sin(phi)) sin(theta),
+    x = (r sin(phi) + a cos(phi)) sin(theta); the noise's azimuth
+    conversion must return (r, sin theta, cos theta, phi)."""
+    cfg, _ = K.preset("cpu256_a09")
+    cfg.spin = a
+    rng = np.random.default_rng(23)
+    for _ in range(200):
+        r = rng.uniform(1.5, 40.0)
+        th = rng.uniform(0.05, math.pi - 0.05)
+        ph = rng.uniform(-math.pi + 1e-6, math.pi - 1e-6)
+        x = (r * math.sin(ph) + a * math.cos(ph)) * math.sin(th)
+        y = r * math.cos(th)
+        z = (r * math.cos(ph) - a * math.sin(ph)) * math.sin(th)
+        rr, s_th, c_th, phi = O.ks_bl(cfg, x, y, z)
+        assert rr == pytest.approx(r, rel=1e-12)
+        assert s_th == pytest.approx(math.sin(th), abs=1e-12)
+        assert c_th == pytest.approx(math.cos(th), abs=1e-12)
+        assert phi == pytest.approx(ph, abs=1e-11)
+
+
 # ------------------------------------------------------ LUT fetches
 def _tex1d_linear_clamp(table, x):
     """A 1-D texture fetch at normalised coordinate x with linear filtering
