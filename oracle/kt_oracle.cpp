@@ -846,6 +846,21 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
 }
 
 // cuda/kernel.cu:99-141 -- jittered pinhole rays of one pixel.
+// Sample sidx of pixel (x, y): the jittered pinhole direction (not
+// normalised) u rgt + v up + fwd, u = ((x + rx) 2/W - 1) aspect fov_scale,
+// v = (1 - (y + ry) 2/H) fov_scale, with rx, ry from the pixel's hash
+// (cuda/kernel.cu:112-121).
+template <class S>
+void pixel_dir(const S fwd[3], const S rgt[3], const S up[3], S fov_scale, int x, int y, uint32_t pixel_index,
+               int sidx, const Params<S>& p, S dir[3]) {
+    uint64_t base = ((uint64_t)pixel_index << 32) | (uint64_t)(uint32_t)sidx;
+    double rx = rand01(base);
+    double ry = rand01(base ^ 0xda3e39cb94b95bdbULL);
+    S u = (((double)x + rx) * p.inv_w_2 - 1.0) * p.aspect * fov_scale;
+    S v = (1.0 - ((double)y + ry) * p.inv_h_2) * fov_scale;
+    for (int c = 0; c < 3; ++c) dir[c] = u * rgt[c] + v * up[c] + fwd[c];
+}
+
 template <class S>
 void trace_pixel(const S cam_pos[3], const S fwd[3], const S rgt[3], const S up[3], S fov_scale, int x, int y,
                  const Params<S>& p, S rgb[3], int* cls, RayStats& st, bool* overflow, double* overflow_T) {
@@ -853,13 +868,8 @@ void trace_pixel(const S cam_pos[3], const S fwd[3], const S rgt[3], const S up[
     uint32_t pixel_index = (uint32_t)(y * p.W + x);
     int cls_all = 0;
     for (int sidx = 0; sidx < p.spp; ++sidx) {
-        uint64_t base = ((uint64_t)pixel_index << 32) | (uint64_t)(uint32_t)sidx;
-        double rx = rand01(base);
-        double ry = rand01(base ^ 0xda3e39cb94b95bdbULL);
-        S u = (((double)x + rx) * p.inv_w_2 - 1.0) * p.aspect * fov_scale;
-        S v = (1.0 - ((double)y + ry) * p.inv_h_2) * fov_scale;
         S dir[3];
-        for (int c = 0; c < 3; ++c) dir[c] = u * rgt[c] + v * up[c] + fwd[c];
+        pixel_dir(fwd, rgt, up, fov_scale, x, y, pixel_index, sidx, p, dir);
         RayOut<S> ro;
         trace_ray(cam_pos, dir, p, ro, st);
         for (int c = 0; c < 3; ++c) acc[c] = acc[c] + ro.col[c];
@@ -1080,6 +1090,15 @@ int kto_sky(const kt_config* cfg, const double d[3], double T, double* intensity
     bool g = false;
     *intensity = sky_intensity(d[0], d[1], d[2], T, p, &g);
     *grid = g ? 1 : 0;
+    return KT_OK;
+}
+
+// The pinhole direction of sample sidx of pixel (x, y) (not normalised).
+int kto_pixel_dir(const kt_config* cfg, const kt_camera* cam, int x, int y, int sidx, double dir[3]) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    pixel_dir(cam->fwd, cam->rgt, cam->up, cam->fov_scale, x, y, (uint32_t)(y * p.W + x), sidx, p, dir);
     return KT_OK;
 }
 

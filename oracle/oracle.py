@@ -75,13 +75,14 @@ def lib():
         L.kto_disk_redshift.argtypes = [C.POINTER(KtConfig), pd, d, d, pd, C.POINTER(C.c_int)]
         L.kto_fetch_disk.argtypes = [C.POINTER(KtConfig), d, pd]
         L.kto_ks_bl.argtypes = [C.POINTER(KtConfig), pd, pd]
+        L.kto_pixel_dir.argtypes = [C.POINTER(KtConfig), C.POINTER(KtCamera), i32, i32, i32, pd]
         L.kto_fetch_blackbody.argtypes = [C.POINTER(KtConfig), d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
                      "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
                      "kto_step_tol", "kto_disk_redshift", "kto_fetch_disk", "kto_fetch_blackbody",
-                     "kto_ks_bl"):
+                     "kto_ks_bl", "kto_pixel_dir"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -300,6 +301,13 @@ def disk_redshift(cfg, x, z, px, py, pz, r, pt, kt):
     o = C.c_int()
     _check(lib().kto_disk_redshift(C.byref(cfg), _dptr(h), pt, kt, C.byref(g), C.byref(o)))
     return g.value, bool(o.value)
+
+
+def pixel_dir(cfg, cam, x, y, sample=0):
+    """The (not normalised) pinhole direction of a pixel's sample."""
+    out = np.zeros(3, dtype=np.float64)
+    _check(lib().kto_pixel_dir(C.byref(cfg), C.byref(cam), x, y, sample, _dptr(out)))
+    return out
 
 
 def ks_bl(cfg, x, y, z):

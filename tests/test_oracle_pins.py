@@ -626,6 +626,34 @@ def test_step_tolerance_closed_form(floor):
     assert seen == {"far", "near", "ramp"}
 
 
+def test_pinhole_geometry():
+    """The pixel directions of a W x H frame with vertical half-angle
+    atan(fov_scale): every sample lies on the image plane at unit distance
+    along fwd, inside its own pixel's slab (column x spans
+    [x, x + 1) / W of the width from the left, row y spans [y, y + 1) / H
+    of the height from the top), the vertical extent is +-fov_scale and
+    the horizontal +-aspect fov_scale; the jitter of different samples of a
+    pixel differs."""
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg, cam = K.scaled(cfg, cam, 64, 36)
+    W, H, f = cfg.width, cfg.height, cam.fov_scale
+    fwd, rgt, up = (np.array(v, dtype=np.float64) for v in (cam.fwd, cam.rgt, cam.up))
+    np.testing.assert_allclose(np.array([fwd, rgt, up]) @ np.array([fwd, rgt, up]).T, np.eye(3), atol=1e-15)
+    us, vs = [], []
+    for y in range(H):
+        for x in range(W):
+            d = O.pixel_dir(cfg, cam, x, y)
+            assert d @ fwd == pytest.approx(1.0, abs=1e-14)
+            u = (d @ rgt) / (f * W / H)   # in [-1, 1] left to right
+            v = (d @ up) / f              # in [-1, 1] bottom to top
+            assert x / W - 1e-12 <= (u + 1) / 2 < (x + 1) / W + 1e-12
+            assert y / H - 1e-12 <= (1 - v) / 2 < (y + 1) / H + 1e-12
+            us.append(u)
+            vs.append(v)
+    assert min(us) < -0.95 and max(us) > 0.95 and min(vs) < -0.9 and max(vs) > 0.9
+    assert not np.allclose(O.pixel_dir(cfg, cam, 5, 7, 0), O.pixel_dir(cfg, cam, 5, 7, 1))
+
+
 @pytest.mark.parametrize("a", [0.0, 0.5, -0.9, 0.998])
 def test_ks_to_bl_inverts_the_coordinate_map(a):
     """Kerr-Schild Cartesian coordinates of a Boyer-Lindquist point (spin
