@@ -626,6 +626,32 @@ def test_step_tolerance_closed_form(floor):
     assert seen == {"far", "near", "ramp"}
 
 
+@pytest.mark.parametrize("max_attempts,tol", [(3, 1e-9), (1, 1e-13), (2, 1e-13)])
+def test_step_loop_bookkeeping(max_attempts, tol):
+    """The integrator's work per ray follows from the step loop's rules
+    (cuda/kerr_trace.cuh:62-134): a step ends on an accepted attempt or
+    after max_attempts rejections (then committed anyway), the accepted
+    step's k7 is the next step's k1 (FSAL), so right-hand sides = 6 per
+    attempt + 1 for the first step + 1 after every forced step + 1 for the
+    escape direction; and a step never takes more than max_attempts
+    attempts."""
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg, cam = K.scaled(cfg, cam, 32, 18)
+    cfg.max_attempts, cfg.tolerance = max_attempts, tol
+    forced_seen = 0
+    for x, y in [(0, 0), (16, 9), (10, 5), (31, 17), (20, 8)]:
+        d = O.pixel_dir(cfg, cam, x, y)
+        _, cls, st, log = O.trace_dir(cfg, list(cam.pos), d, max_log=200000)
+        escaped = 1 if (cls & 3) == 2 else 0
+        forced = st["steps"] - st["accepted"]
+        forced_seen += forced
+        assert st["rhs"] == 6 * st["attempts"] + 1 + forced + escaped
+        assert st["accepted"] <= st["attempts"] <= max_attempts * st["steps"]
+        assert len(log) == st["steps"] and int(log[:, 1].sum()) == st["accepted"]
+    if max_attempts == 1:
+        assert forced_seen > 0
+
+
 def test_pinhole_geometry():
     """The pixel directions of a W x H frame with vertical half-angle
     atan(fov_scale): every sample lies on the image plane at unit distance
