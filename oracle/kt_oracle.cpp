@@ -600,9 +600,14 @@ void dp_attempt(const State<S>& s, S pt, S hs, double tol, const State<S>& k1, c
 }
 
 // Per committed step: h used, accepted flag, state after the step.
+// Per equatorial crossing (optional, kCrossRec values each): the pieces of
+// the disk emission, so tests can check how they are assembled.
+static constexpr int kCrossRec = 24;
 struct StepLog {
     std::vector<double> rows;  // 8 values per step
     size_t max_steps = 0;
+    std::vector<double> cross;  // kCrossRec values per crossing
+    size_t max_cross = 0;
 };
 
 template <class S>
@@ -778,15 +783,32 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
             S py_hit = dense(prev.py, s.py, k1.py, k3.py, k4.py, k5.py, k6.py, k7.py, h_used, t);
             S pz_hit = dense(prev.pz, s.pz, k1.pz, k3.pz, k4.pz, k5.pz, k6.pz, k7.pz, h_used, t);
             S r_hit = ks_r(x_hit, S(0.0), z_hit, p.aa);
+            // crossing record: t, hit state, r, in-disk, orbit, g, u, LUT,
+            // T before/after noise, rgb, alpha, Tr before/after, BL r, phi,
+            // modulator, composited (NaN where a stage was not reached)
+            double rec[kCrossRec];
+            const bool want_rec = log && log->cross.size() / kCrossRec < log->max_cross;
+            if (want_rec) {
+                for (int q = 0; q < kCrossRec; ++q) rec[q] = std::nan("");
+                rec[0] = val(t); rec[1] = val(x_hit); rec[2] = val(z_hit); rec[3] = val(px_hit);
+                rec[4] = val(py_hit); rec[5] = val(pz_hit); rec[6] = val(r_hit);
+                rec[7] = (r_hit >= p.disk_inner && r_hit <= p.disk_outer) ? 1.0 : 0.0;
+                rec[8] = 0.0;
+                rec[18] = val(transmittance);
+                rec[19] = val(transmittance);
+                rec[23] = 0.0;
+            }
             if (r_hit >= p.disk_inner && r_hit <= p.disk_outer) {
                 // Keplerian disk emission (kerr_trace.cuh:177-199)
                 bool orbit = false;
                 S g = disk_redshift(x_hit, z_hit, px_hit, py_hit, pz_hit, pt, kt, r_hit, p, orbit);
+                if (want_rec) rec[8] = orbit ? 1.0 : 0.0;
                 if (orbit) {
                     S dspan = p.disk_outer - p.disk_inner;
                     S uu = dspan > 0.0 ? (r_hit - p.disk_inner) / dspan : S(0.0);
                     S sampled = fetch_disk(uu, p);
                     S T = p.T_scale * sampled * g;
+                    if (want_rec) { rec[9] = val(g); rec[10] = val(uu); rec[11] = val(sampled); rec[12] = val(T); }
                     if (p.noise_on) {
                         // kerr_trace.cuh:200-214 (ks_bl overwrites r_hit)
                         S sin_th, cos_th, phi_hit;
@@ -800,7 +822,9 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
                         S nval = fbm(nu, nv, p.noise_detail, p.noise_seed);
                         S modulator = 1.0 + p.noise_strength * (nval - 0.5);
                         T = T * modulator;
+                        if (want_rec) { rec[20] = val(r_hit); rec[21] = val(phi_hit); rec[22] = val(modulator); }
                     }
+                    if (want_rec) rec[13] = val(T);
                     if (std::isfinite(val(T)) && T > 0.0) {
                         // kerr_trace.cuh:215-225
                         S d_col[3];
@@ -816,9 +840,15 @@ void trace_ray(const S cam[3], const S dir[3], const Params<S>& p, RayOut<S>& ou
                             cls |= KT_FLAG_DISK;
                             if (((cls >> 5) & 7) < 7) cls += 32;  // contributions, saturating at 7
                         }
+                        if (want_rec) {
+                            rec[14] = val(d_col[0]); rec[15] = val(d_col[1]); rec[16] = val(d_col[2]);
+                            rec[17] = val(alpha); rec[19] = val(transmittance);
+                            rec[23] = val(alpha) > 0.0 ? 1.0 : 0.0;
+                        }
                     }
                 }
             }
+            if (want_rec) log->cross.insert(log->cross.end(), rec, rec + kCrossRec);
         }
         if (r_now > p.R_esc) {
             // escape: sky colour along normalize(dx/dl) (kerr_trace.cuh:229-237)
@@ -1188,6 +1218,36 @@ int kto_trace_dir(const kt_config* cfg, const double cam[3], const double dir[3]
     if (log) {
         std::memcpy(log, L.rows.data(), L.rows.size() * sizeof(double));
         if (n_log) *n_log = (int)(L.rows.size() / 8);
+    }
+    return ro.cls;
+}
+
+// As kto_trace_dir, also returning the ray's equatorial crossings: at most
+// max_cross records of 24 doubles (t, x, z, px, py, pz at the hit, KS r,
+// in-disk flag, orbit flag, redshift g, LUT coordinate u, disk LUT value,
+// T before noise, T after noise, rgb, alpha, transmittance before and
+// after, BL r and phi of the noise, modulator, composited flag; NaN where
+// the shading stopped earlier); *n_cross receives the number written.  pk
+// (optional) receives the ray's (pt, kt).
+int kto_trace_dir_x(const kt_config* cfg, const double cam[3], const double dir[3], double col[3],
+                    double* cross, int max_cross, int* n_cross, double pk[2]) {
+    Params<double> p;
+    int rc = make_params(cfg, p);
+    if (rc != KT_OK) return rc;
+    RayOut<double> ro;
+    RayStats st;
+    StepLog L;
+    L.max_cross = (size_t)std::max(max_cross, 0);
+    trace_ray(cam, dir, p, ro, st, &L);
+    for (int c = 0; c < 3; ++c) col[c] = ro.col[c];
+    std::memcpy(cross, L.cross.data(), L.cross.size() * sizeof(double));
+    if (n_cross) *n_cross = (int)(L.cross.size() / kCrossRec);
+    if (pk) {
+        State<double> s0;
+        double pt = 0.0, kt = 0.0;
+        init_ray(cam, dir, p, s0, pt, kt);
+        pk[0] = pt;
+        pk[1] = kt;
     }
     return ro.cls;
 }

@@ -79,8 +79,9 @@ def lib():
         L.kto_fetch_blackbody.argtypes = [C.POINTER(KtConfig), d, pd, C.POINTER(C.c_int)]
         L.kto_trace_dir.argtypes = [C.POINTER(KtConfig), pd, pd, pd, C.POINTER(C.c_uint32), pd, i32,
                                     C.POINTER(C.c_int)]
+        L.kto_trace_dir_x.argtypes = [C.POINTER(KtConfig), pd, pd, pd, pd, i32, C.POINTER(C.c_int), pd]
         for name in ("kto_derive", "kto_get_derivs", "kto_init_ray", "kto_render_rows", "kto_post",
-                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_sky",
+                     "kto_flop_counts", "kto_dp_attempt", "kto_trace_dir", "kto_trace_dir_x", "kto_sky",
                      "kto_step_tol", "kto_disk_redshift", "kto_fetch_disk", "kto_fetch_blackbody",
                      "kto_ks_bl", "kto_pixel_dir"):
             getattr(L, name).restype = C.c_int
@@ -364,3 +365,25 @@ def trace_dir(cfg, cam_pos, direction, max_log=0):
         _check(rc)
     stats = dict(zip(("attempts", "rhs", "accepted", "crossings", "steps"), list(st)))
     return col, rc, stats, log[:n.value].copy()
+
+
+# Fields of a crossing record (kto_trace_dir_x), in order.
+CROSS_FIELDS = ("t", "x", "z", "px", "py", "pz", "r", "in_disk", "orbit", "g", "u", "lut", "T_base", "T",
+                "r_col", "g_col", "b_col", "alpha", "tr_before", "tr_after", "r_bl", "phi", "modulator",
+                "composited")
+
+
+def trace_dir_crossings(cfg, cam_pos, direction, max_cross=64):
+    """One ray: (rgb before exposure, class byte, crossing records [n, 24]
+    with columns CROSS_FIELDS, (pt, kt) of the ray)."""
+    cam = np.ascontiguousarray(cam_pos, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    col = np.zeros(3, dtype=np.float64)
+    rec = np.zeros((max(max_cross, 1), len(CROSS_FIELDS)), dtype=np.float64)
+    pk = np.zeros(2, dtype=np.float64)
+    n = C.c_int(0)
+    rc = lib().kto_trace_dir_x(C.byref(cfg), _dptr(cam), _dptr(d), _dptr(col), _dptr(rec), max_cross,
+                               C.byref(n), _dptr(pk))
+    if rc < 0:
+        _check(rc)
+    return col, rc, rec[:n.value].copy(), (pk[0], pk[1])
