@@ -36,11 +36,11 @@ PEAK_FP32_VECTOR = 157.3e12  # MI355X FP32 vector FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
 
 
-def pmc_traffic():
-    """HBM bytes per tracer launch measured with rocprofv3 --pmc FETCH_SIZE
-    and WRITE_SIZE on this workload (profiles/, with the commands that made
-    them), or None."""
-    path = os.path.join(ROOT, "profiles", "trace_fp64_hd1080_a0998_traffic.json")
+def pmc_traffic(name="trace_fp64_hd1080_a0998_traffic.json"):
+    """HBM bytes per launch (tracer) or per frame (post passes) measured with
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload (profiles/,
+    with the commands that made them), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -383,6 +383,21 @@ def grad_leg(args):
     return out
 
 
+def copy_calibration(mib, reps=3):
+    """A device-to-device copy of known size: under rocprofv3 --pmc
+    FETCH_SIZE (then WRITE_SIZE) each copy dispatch should report mib * 1024
+    KiB; the ratio is the scale applied to the tracer's and the post
+    passes' counts."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copied {mib} MiB x {reps} ({mib * 1024} KiB read and written per copy)", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,7 +418,14 @@ def main():
     ap.add_argument("--streams", type=int, default=0,
                     help="HIP streams of the streamed frames, 1 to 3 (default: tuned between 2 and 3; "
                          "2 with --launch-opts or --no-autotune)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB three times and exit: the "
+                         "calibration of the FETCH_SIZE / WRITE_SIZE counters (a copy of N MiB reads "
+                         "and writes N MiB)")
     args = ap.parse_args()
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+        return
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
                  "multi-GPU runs are not part of this build (use --gpus 1)")
@@ -499,6 +521,7 @@ def main():
     t_post = sum(post_ms) / len(post_ms) / 1e3
     flops = trace_flops(st, fc)
     traffic = pmc_traffic()
+    post_traffic = pmc_traffic("post_hd1080_a0998_traffic.json")
     achieved = flops / t_kernel
     post_bytes = 20.0 * W * H  # compulsory: float4 HDR read + u32 write per pixel
     out = {
@@ -531,9 +554,9 @@ def main():
                  "crossings": st["crossings"], "max_attempts_ray": st["max_attempts_ray"],
                  "attempts_per_ray": round(st["attempts"] / rays, 3),
                  "kernel_ms": round(st["kernel_us"] / 1e3, 4), "tail_ms": round(st["tail_us"] / 1e3, 4)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
-                     "compute_unit": "FP64 VALU (the tracer has no matrix-shaped work and issues no MFMA; MI355X's "
-                                     "FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)",
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 4), "peak": PEAK_FP64_VECTOR / 1e12,
+                     "compute_unit": "FP64 VALU: the tracer has no matrix-shaped work and issues no MFMA; priced "
+                                     "against MI355X's FP64 vector peak",
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_VECTOR, 4),
                      "traffic": (traffic or {}).get("bytes_per_launch"),
                      "kernel": "trace_fp64_persistent",
@@ -541,7 +564,9 @@ def main():
                      "streamed_achieved": round(flops / (ms_per_step / 1e3) / 1e12, 4),
                      "streamed_frac": round(flops / (ms_per_step / 1e3) / PEAK_FP64_VECTOR, 4)},
         "post_roofline": {"bound": "hbm", "achieved": round(post_bytes / t_post / 1e9, 2), "peak": PEAK_HBM / 1e9,
-                          "unit": "GB/s", "frac": round(post_bytes / t_post / PEAK_HBM, 4), "traffic": None,
+                          "unit": "GB/s", "frac": round(post_bytes / t_post / PEAK_HBM, 4),
+                          "traffic": (post_traffic or {}).get("bytes_per_frame"),
+                          "traffic_over_compulsory": (post_traffic or {}).get("traffic_over_compulsory"),
                           "kernels": "bloom_h + post_v"},
     }
     print(f"[bench] timed {args.steps} frames: {latency_ms:.2f} ms/frame one at a time, {ms_per_step:.2f} ms/frame "

@@ -551,7 +551,8 @@ struct kt_grad {
     hipEvent_t ev[4] = {};
     hipEvent_t ev_params = nullptr;
     bool params_pending = false, have_forward = false, have_backward = false;
-    hipStream_t last_stream = nullptr;
+    hipStream_t last_stream = nullptr;  // stream of the last forward or backward
+    hipStream_t fwd_stream = nullptr;   // stream of the last forward (its class copy runs there)
     uint64_t items = 0;
 };
 
@@ -1273,7 +1274,10 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
 
 void kt_grad_destroy(kt_grad* g) {
     if (!g) return;
-    if (g->last_stream || g->have_forward) (void)hipStreamSynchronize(g->last_stream);
+    if (g->have_forward) {
+        (void)hipStreamSynchronize(g->fwd_stream);
+        (void)hipStreamSynchronize(g->last_stream);
+    }
     release_grad(g);
 }
 
@@ -1284,9 +1288,12 @@ kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr, uint8_t
     if (!(theta[2] > 0.0) || !(theta[3] > 0.0))
         return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "r_obs and mdot must be positive");
     hipStream_t s = (hipStream_t)stream;
-    // the previous frame's work reads the buffers rewritten below
+    // the previous frame's work reads the buffers rewritten below: its
+    // forward (trace, class copy) and its backward, which may have run on
+    // another stream
     if (g->have_forward) {
-        hipError_t e = hipStreamSynchronize(g->last_stream);
+        hipError_t e = hipStreamSynchronize(g->fwd_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->last_stream);
         if (e != hipSuccess) return hip_err(nullptr, e, "previous frame wait");
     }
     std::string why;
@@ -1344,6 +1351,7 @@ kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr, uint8_t
         if (e != hipSuccess) return hip_err(nullptr, e, "class copy");
     }
     g->last_stream = s;
+    g->fwd_stream = s;
     g->have_forward = true;
     g->have_backward = false;
     return KT_OK;

@@ -57,12 +57,17 @@ class GradRenderer:
     def forward(self, theta, hdr=None, cls=None, stream=None):
         """Trace the frame at theta (4 floats); returns (hdr float32 [H, W, 4],
         cls uint8 [H, W] or None) on the current torch stream."""
+        from .renderer import _check_tensor
         torch = _torch()
         th = (C.c_double * 4)(*[float(v) for v in theta])
         H, W = self.shape
-        if hdr is None:
-            hdr = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
         s = stream if stream is not None else torch.cuda.current_stream()
+        if hdr is None:
+            with torch.cuda.stream(s):
+                hdr = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        _check_tensor(hdr, (H, W, 4), torch.float32)
+        if cls is not None:
+            _check_tensor(cls, (H, W), torch.uint8)
         check(lib().kt_grad_forward(self._h, th, C.c_void_p(hdr.data_ptr()),
                                     C.c_void_p(cls.data_ptr() if cls is not None else 0),
                                     C.c_void_p(s.cuda_stream)))
@@ -74,13 +79,21 @@ class GradRenderer:
         on the device (and the per-pixel contributions [H, W, 4] if asked)."""
         torch = _torch()
         H, W = self.shape
-        dl = dl_dhdr.detach()
-        if tuple(dl.shape) != (H, W, 4) or dl.dtype != torch.float32 or not dl.is_cuda:
-            dl = dl.to(device="cuda", dtype=torch.float32).reshape(H, W, 4)
-        dl = dl.contiguous()
-        grad = torch.empty(4, dtype=torch.float64, device="cuda")
-        gpx = torch.empty((H, W, 4), dtype=torch.float64, device="cuda") if per_pixel else None
         s = stream if stream is not None else torch.cuda.current_stream()
+        cur = torch.cuda.current_stream()
+        dl = dl_dhdr.detach()
+        if dl.numel() != H * W * 4:
+            raise ValueError(f"dl_dhdr must hold {H}x{W}x4 values, got shape {tuple(dl.shape)}")
+        if s != cur:
+            s.wait_stream(cur)  # dl_dhdr is produced on the current stream
+        with torch.cuda.stream(s):
+            # conversions and outputs are made on s, so their memory is
+            # ordered with the adjoint kernels that use it
+            dl = dl.to(device="cuda", dtype=torch.float32).reshape(H, W, 4).contiguous()
+            grad = torch.empty(4, dtype=torch.float64, device="cuda")
+            gpx = torch.empty((H, W, 4), dtype=torch.float64, device="cuda") if per_pixel else None
+        if dl.data_ptr() == dl_dhdr.data_ptr() and s != cur:
+            dl_dhdr.record_stream(s)  # the caller's own buffer is read on s
         check(lib().kt_grad_backward(self._h, C.c_void_p(dl.data_ptr()), C.c_void_p(grad.data_ptr()),
                                      C.c_void_p(gpx.data_ptr() if gpx is not None else 0),
                                      C.c_void_p(s.cuda_stream)))

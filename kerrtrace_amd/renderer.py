@@ -83,9 +83,11 @@ class HipRenderer:
         """Pick the fastest launch options for this frame (kt_autotune);
         returns them as a dict.  hdr is overwritten."""
         torch = _torch()
-        if hdr is None:
-            hdr = self.alloc_hdr()
         s = stream if stream is not None else torch.cuda.current_stream()
+        if hdr is None:
+            with torch.cuda.stream(s):
+                hdr = self.alloc_hdr()
+        _check_tensor(hdr, (self.height, self.width, 4), torch.float32)
         o = KtLaunchOpts()
         check(lib().kt_autotune(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()), C.c_void_p(s.cuda_stream),
                                 C.byref(o)), self._h)
@@ -96,18 +98,20 @@ class HipRenderer:
         return torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
 
     def trace(self, cam: KtCamera, hdr=None, cls=None, with_class=True, stream=None, attempts=None):
-        """Trace one frame on the current torch stream.  Returns (hdr, cls):
-        float32 [H, W, 4] RGBA (A = 1) and uint8 [H, W] hit classes.
+        """Trace one frame on `stream` (default: the current torch stream).
+        Returns (hdr, cls): float32 [H, W, 4] RGBA (A = 1) and uint8 [H, W]
+        hit classes; tensors it allocates belong to that stream.
         attempts: optional int32 [H, W] tensor receiving per-pixel attempts."""
         torch = _torch()
-        if hdr is None:
-            hdr = self.alloc_hdr()
-        if cls is None and with_class:
-            cls = torch.empty((self.height, self.width), dtype=torch.uint8, device="cuda")
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            if hdr is None:
+                hdr = self.alloc_hdr()
+            if cls is None and with_class:
+                cls = torch.empty((self.height, self.width), dtype=torch.uint8, device="cuda")
         _check_tensor(hdr, (self.height, self.width, 4), torch.float32)
         if cls is not None:
             _check_tensor(cls, (self.height, self.width), torch.uint8)
-        s = stream if stream is not None else torch.cuda.current_stream()
         if attempts is not None:
             _check_tensor(attempts, (self.height, self.width), torch.int32)
             check(lib().kt_trace_ex(self._h, C.byref(cam), C.c_void_p(hdr.data_ptr()),
@@ -126,14 +130,20 @@ class HipRenderer:
         return {name: getattr(st, name) for name, _ in st._fields_}
 
     def post_process(self, hdr, ldr=None, bloom=None, stream=None):
-        """Bloom + ACES + sRGB -> int32 [H, W] holding 0x00RRGGBB."""
+        """Bloom + ACES + sRGB -> int32 [H, W] holding 0x00RRGGBB, on
+        `stream` (default: the current torch stream).  bloom: float32
+        [H, W, 4] scratch, allocated on that stream when not given."""
         torch = _torch()
         _check_tensor(hdr, (self.height, self.width, 4), torch.float32)
-        if ldr is None:
-            ldr = torch.empty((self.height, self.width), dtype=torch.int32, device="cuda")
-        if bloom is None and self.derived.bloom_active:
-            bloom = torch.empty_like(hdr)
         s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            if ldr is None:
+                ldr = torch.empty((self.height, self.width), dtype=torch.int32, device="cuda")
+            if bloom is None and self.derived.bloom_active:
+                bloom = torch.empty_like(hdr)
+        _check_tensor(ldr, (self.height, self.width), torch.int32)
+        if bloom is not None:
+            _check_tensor(bloom, (self.height, self.width, 4), torch.float32)
         check(lib().kt_post_process(self._h, C.c_void_p(hdr.data_ptr()),
                                     C.c_void_p(bloom.data_ptr() if bloom is not None else 0),
                                     C.c_void_p(ldr.data_ptr()), C.c_void_p(s.cuda_stream)), self._h)
@@ -223,11 +233,16 @@ class HipBatch:
         """Pick the fastest launch options for the batch (kt_batch_autotune);
         returns them as a dict.  The hdr tensors are overwritten."""
         torch = _torch()
-        if hdr is None:
-            hdr, _ = self.alloc(with_class=False)
-        n = len(self.cfgs)
-        hp = (C.c_void_p * n)(*[t.data_ptr() for t in hdr])
         s = stream if stream is not None else torch.cuda.current_stream()
+        if hdr is None:
+            with torch.cuda.stream(s):
+                hdr, _ = self.alloc(with_class=False)
+        n = len(self.cfgs)
+        if len(hdr) != n:
+            raise ValueError(f"expected {n} output tensors")
+        for i, c in enumerate(self.cfgs):
+            _check_tensor(hdr[i], (c.height, c.width, 4), torch.float32)
+        hp = (C.c_void_p * n)(*[t.data_ptr() for t in hdr])
         o = KtLaunchOpts()
         check(lib().kt_batch_autotune(self._h, hp, C.c_void_p(s.cuda_stream), C.byref(o)))
         return {name: getattr(o, name) for name, _ in o._fields_}
@@ -243,8 +258,10 @@ class HipBatch:
         """Trace every render on the current torch stream; returns lists of
         float32 [H, W, 4] and (optionally) uint8 [H, W] tensors."""
         torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
         if hdr is None:
-            hdr, c2 = self.alloc(with_class and cls is None)
+            with torch.cuda.stream(s):
+                hdr, c2 = self.alloc(with_class and cls is None)
             cls = cls if cls is not None else c2
         n = len(self.cfgs)
         if len(hdr) != n or (cls is not None and len(cls) != n):
@@ -255,7 +272,6 @@ class HipBatch:
                 _check_tensor(cls[i], (c.height, c.width), torch.uint8)
         hp = (C.c_void_p * n)(*[t.data_ptr() for t in hdr])
         cp = (C.c_void_p * n)(*[t.data_ptr() for t in cls]) if cls is not None else None
-        s = stream if stream is not None else torch.cuda.current_stream()
         check(lib().kt_batch_trace(self._h, hp, cp, C.c_void_p(s.cuda_stream)))
         return hdr, cls
 
@@ -266,8 +282,12 @@ class HipBatch:
 
 
 def _check_tensor(t, shape, dtype):
-    if tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
-        raise ValueError(f"expected a contiguous cuda tensor {shape} {dtype}, got {tuple(t.shape)} {t.dtype}")
+    """A caller's buffer goes to the kernels as a raw pointer: reject any
+    that is not exactly the contiguous device tensor they index."""
+    if (not hasattr(t, "shape") or tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_cuda
+            or not t.is_contiguous()):
+        got = f"{tuple(t.shape)} {t.dtype}" if hasattr(t, "shape") else type(t).__name__
+        raise ValueError(f"expected a contiguous cuda tensor {shape} {dtype}, got {got}")
 
 
 def unpack_ldr(ldr):

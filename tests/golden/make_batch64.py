@@ -1,11 +1,11 @@
-"""Writes batch64_seed7.json: 64 square renders drawn once with
+"""Writes kerrtrace_amd/presets/batch64_seed7.json: 64 square renders drawn once with
 numpy.random.default_rng(7).  Per render, in this order:
     spin        ~ U[0, 0.998]
     inclination ~ U[5, 89] deg
     side        = round(exp(U[ln 128, ln 1024])) px  (W = H = side)
     r_obs       ~ U[30, 200] M
 The JSON is committed, so loading the batch needs no generator.
-Run: python kerrtrace_amd/presets/make_batch64.py
+Run: python tests/golden/make_batch64.py
 """
 import json
 import math
@@ -39,7 +39,8 @@ def main():
         "total_rays": sum(r["side"] ** 2 for r in renders),
         "renders": renders,
     }
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "batch64_seed7.json")
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    path = os.path.join(root, "kerrtrace_amd", "presets", "batch64_seed7.json")
     with open(path, "w", encoding="utf-8") as fh:
         json.dump(doc, fh, indent=1)
         fh.write("\n")

@@ -162,3 +162,19 @@ def test_null_arguments_return_errors():
     assert L.kt_trace(None, None, None, None, None) == _abi.KT_ERR_INVALID_ARGUMENT
     assert L.kt_get_trace_stats(None, None) == _abi.KT_ERR_INVALID_ARGUMENT
     L.kt_destroy(None)
+
+
+def test_buffer_guard_rejects_mismatched_tensors():
+    """Caller buffers reach the kernels as raw pointers, so the Python host
+    refuses any tensor that is not the exact contiguous device array a
+    kernel indexes (CPU tensors stand in here; on a GPU box the same guard
+    runs on device tensors)."""
+    import torch
+
+    from kerrtrace_amd.renderer import _check_tensor
+    good = torch.empty((4, 6, 4), dtype=torch.float32)
+    for t, shape, dt in [(good, (4, 6, 3), torch.float32), (good, (4, 6, 4), torch.float64),
+                         (good.transpose(0, 1), (6, 4, 4), torch.float32), (good, (4, 6, 4), torch.float32),
+                         ("not a tensor", (4, 6, 4), torch.float32)]:
+        with pytest.raises(ValueError):
+            _check_tensor(t, shape, dt)

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_batch64_fixture_matches_generator():
-    path = os.path.join(ROOT, "kerrtrace_amd", "presets", "make_batch64.py")
+    path = os.path.join(ROOT, "tests", "golden", "make_batch64.py")
     spec = importlib.util.spec_from_file_location("make_batch64", path)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)

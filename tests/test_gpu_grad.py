@@ -166,3 +166,15 @@ def test_backward_on_another_stream(torch_gpu):
             got = g.backward(w2, stream=s2)
             s2.synchronize()
             assert torch.equal(got.cpu(), ref)
+        # a float64, non-contiguous loss gradient made on the current stream
+        # is converted on the adjoint's stream, after the current stream's work
+        w64 = w.to(torch.float64).transpose(0, 1).contiguous().transpose(0, 1)
+        assert not w64.is_contiguous()
+        g.forward(THETA, stream=s1)
+        got = g.backward(w64, stream=s2)
+        s2.synchronize()
+        assert torch.equal(got.cpu(), ref)
+        with pytest.raises(ValueError):
+            g.backward(w[:, :-1])
+        with pytest.raises(ValueError):
+            g.forward(THETA, hdr=torch.empty((cfg.height, cfg.width, 3), device="cuda"))

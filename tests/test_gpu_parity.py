@@ -165,6 +165,39 @@ def test_post_process_matches_oracle(torch_gpu):
     assert d <= 1
 
 
+def test_post_process_on_another_stream_and_bad_buffers(torch_gpu):
+    """post_process(stream=s) with library-made bloom and image buffers
+    gives the same image as on the current stream; wrongly shaped or typed
+    caller buffers are refused before any launch."""
+    torch = torch_gpu
+    from kerrtrace_amd.renderer import HipRenderer
+    cfg, cam = K.preset("cpu256_a09")
+    cfg, cam = K.scaled(cfg, cam, 160, 96)
+    say("[parity] post-process on a side stream")
+    with HipRenderer(cfg) as r:
+        hdr, _ = r.trace(cam, with_class=False)
+        ref = r.post_process(hdr)
+        torch.cuda.synchronize()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        for _ in range(3):
+            got = r.post_process(hdr, stream=s)
+            s.synchronize()
+            assert torch.equal(got, ref)
+        H, W = cfg.height, cfg.width
+        bad = [dict(ldr=torch.empty((H, W - 1), dtype=torch.int32, device="cuda")),
+               dict(ldr=torch.empty((H, W), dtype=torch.float32, device="cuda")),
+               dict(bloom=torch.empty((H, W, 3), dtype=torch.float32, device="cuda")),
+               dict(bloom=torch.empty((H, W, 4), dtype=torch.float64, device="cuda"))]
+        for kw in bad:
+            with pytest.raises(ValueError):
+                r.post_process(hdr, **kw)
+        with pytest.raises(ValueError):
+            r.post_process(hdr[:, :-1])
+        with pytest.raises(ValueError):
+            r.autotune(cam, hdr=torch.empty((H, W, 4), dtype=torch.float16, device="cuda"))
+
+
 def test_post_process_no_bloom_and_threshold(torch_gpu):
     torch = torch_gpu
     from kerrtrace_amd.renderer import HipRenderer
