@@ -257,6 +257,17 @@ kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out);
 /* bloom: float4[W*H] scratch; ldr: uint32[W*H] 0x00RRGGBB. */
 kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom,
                           uint32_t* ldr, void* stream);
+/* As kt_post_process, with the bloom intermediate in a device array the
+ * library holds itself (32 MiB per device, kept until the process ends)
+ * instead of a caller buffer.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  It calls no allocator.  Frames of up to 2,097,152
+ * pixels (1920x1080 fits, 3840x2160 does not: KT_ERR_CAPACITY) when bloom
+ * is on; any size when it is off.  All calls on one device share the
+ * array: calls that overlap in time on one device (other streams, other
+ * renderers, other threads) race and must be ordered by the caller. */
+kt_status kt_post_process_noalloc(kt_renderer* r, const float* hdr,
+                                  uint32_t* ldr, void* stream);
 /* Host copies of the lookup tables the device uses (for parity checks):
  * disk: double[lut_size]; blackbody: double[lut_size*3] (rgb). */
 kt_status kt_get_luts(const kt_renderer* r, double* disk_lut, double* bb_lut);

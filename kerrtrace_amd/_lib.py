@@ -34,6 +34,7 @@ SIGNATURES = {
     "kt_set_launch_opts": ([_vp, C.POINTER(KtLaunchOpts)], _i32),
     "kt_get_trace_stats": ([_vp, C.POINTER(KtTraceStats)], _i32),
     "kt_post_process": ([_vp, _vp, _vp, _vp, _vp], _i32),
+    "kt_post_process_noalloc": ([_vp, _vp, _vp, _vp], _i32),
     "kt_get_luts": ([_vp, _pd, _pd], _i32),
     "kt_batch_create": ([C.POINTER(KtConfig), C.POINTER(KtCamera), _i32, C.POINTER(_vp)], _i32),
     "kt_batch_destroy": ([_vp], None),

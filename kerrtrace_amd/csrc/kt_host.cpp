@@ -51,6 +51,8 @@ hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& 
                                 uint32_t n_pix, double* part, double* grad_out, hipStream_t stream);
 size_t cost_order_hist_words(int n_tiles);
 size_t bloom_h_lds_bytes(int R);
+size_t noalloc_max_pixels();
+hipError_t post_scratch_address(int device, float** out);
 }  // namespace kt
 
 namespace {
@@ -213,6 +215,8 @@ struct kt_renderer {
     float* d_bb_lut_f = nullptr;
     float* d_bloom_w = nullptr;
     float* d_bloom_ws = nullptr;  // [W] column then [H] row sums of in-bounds weights
+    float* d_noalloc_bloom = nullptr;  // the library's static bloom array on this device (not owned)
+    std::string noalloc_err;           // why d_noalloc_bloom could not be resolved
     int post_k = 8, post_waves = 8;  // post_v tile: rows per lane, waves per block
     // Per-launch state in kSlots slots that traces use in turn, so that up
     // to kSlots traces issued on as many streams run concurrently: the
@@ -755,6 +759,16 @@ kt_status kt_create(const kt_config* cfg, kt_renderer** out) {
     KT_CREATE_HIP(hipGetDeviceProperties(&prop, r->device), "hipGetDeviceProperties");
     r->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     r->max_lds = prop.sharedMemPerBlock > 0 ? prop.sharedMemPerBlock : 65536;
+    {
+        // device address of the static bloom array of kt_post_process_noalloc
+        // (resolved once per device; a failure only disables that entry point)
+        hipError_t e = kt::post_scratch_address(r->device, &r->d_noalloc_bloom);
+        if (e != hipSuccess) {
+            r->d_noalloc_bloom = nullptr;
+            r->noalloc_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+            (void)hipGetLastError();
+        }
+    }
     const int n = (int)cfg->lut_size;
     std::vector<float> disk_f(n), bb_f((size_t)n * 3);
     for (int i = 0; i < n; ++i) disk_f[i] = (float)r->d.disk_lut[i];
@@ -915,6 +929,20 @@ kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out) {
 kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom, uint32_t* ldr, void* stream) {
     if (!r || !hdr || !ldr || (r->d.bloom_active && !bloom)) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
     return post_into(r, hdr, bloom, ldr, (hipStream_t)stream);
+}
+
+kt_status kt_post_process_noalloc(kt_renderer* r, const float* hdr, uint32_t* ldr, void* stream) {
+    if (!r || !hdr || !ldr) return set_err(r, KT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!r->d.bloom_active) return post_into(r, hdr, nullptr, ldr, (hipStream_t)stream);
+    const size_t px = (size_t)r->cfg.width * r->cfg.height;
+    if (px > kt::noalloc_max_pixels())
+        return set_err(r, KT_ERR_CAPACITY,
+                       "kt_post_process_noalloc: a " + std::to_string(r->cfg.width) + "x" +
+                           std::to_string(r->cfg.height) + " frame exceeds the library's bloom array of " +
+                           std::to_string(kt::noalloc_max_pixels()) + " pixels (use kt_post_process)");
+    if (!r->d_noalloc_bloom)
+        return set_err(r, KT_ERR_DEVICE, "kt_post_process_noalloc: bloom array unavailable: " + r->noalloc_err);
+    return post_into(r, hdr, r->d_noalloc_bloom, ldr, (hipStream_t)stream);
 }
 
 kt_status kt_submit_render_f64(kt_renderer* r, const kt_camera* cam, int32_t* submitted) {

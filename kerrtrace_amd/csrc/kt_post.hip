@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace kt {
 
 static constexpr int kHThreads = 128;  // bloom_h: threads per block
@@ -226,6 +228,38 @@ bool post_v_plan(int R, size_t max_lds, int* k, int* waves) {
         }
     }
     return false;
+}
+
+// ---- library-owned bloom intermediate (kt_post_process_noalloc)
+// The horizontal pass's output lives here instead of in a caller buffer,
+// for callers that cannot allocate.  bloom_h writes every pixel of the
+// frame before post_v reads it, so nothing is cleared between calls.  In
+// host code `scratch` names a host-side stand-in: kernels get the device
+// address that post_scratch_address resolves per device.
+static constexpr size_t kNoAllocMaxPixels = size_t(1) << 21;  // 2,097,152 px: 32 MiB of float4
+static __device__ __align__(256) float4 scratch[kNoAllocMaxPixels];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+size_t noalloc_max_pixels() { return kNoAllocMaxPixels; }
+
+// Device address of `scratch` on `device`, which must be the current
+// device: resolved once per device with hipGetSymbolAddress (each device
+// holds its own copy of the array), then reused.
+hipError_t post_scratch_address(int device, float** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, const float* d_wts,
