@@ -187,6 +187,7 @@ def mixed_leg(cfg, cam, r64, hdr64, args, fc, t64, launch_opts):
         key = "same_tol" if tol == cfg.tolerance else f"tol_{tol:g}"
         out[key] = {
             "tolerance": tol,
+            "fp32_rhs": tol >= cfg.mixed_min_tolerance,
             "trace_ms": round(tm * 1e3, 4),
             "trace_mrays_s": round(rays / tm / 1e6, 4),
             "fp64_same_tol_trace_ms": round(tf * 1e3, 4),
@@ -398,6 +399,34 @@ def copy_calibration(mib, reps=3):
     print(f"[bench] copied {mib} MiB x {reps} ({mib * 1024} KiB read and written per copy)", flush=True)
 
 
+def trace_only(spec, steps, launch_opts):
+    """Trace one workload `steps` times and print its work counters: the
+    command profiles run under rocprofv3.  spec = preset[:tolerance[:fp64|mixed[:spin]]],
+    e.g. hd1080_a0998:1e-7:mixed or uhd2160_i60:1e-7:fp64:0.998."""
+    import torch
+
+    from kerrtrace_amd import config as K
+    from kerrtrace_amd.renderer import HipRenderer
+    parts = spec.split(":")
+    kw = {"spin": float(parts[3])} if len(parts) > 3 else {}
+    cfg, cam = K.preset(parts[0], **kw)
+    if len(parts) > 1 and parts[1]:
+        cfg.tolerance = float(parts[1])
+    if len(parts) > 2:
+        cfg.precision = {"fp64": K.KT_PRECISION_FP64, "mixed": K.KT_PRECISION_MIXED}[parts[2]]
+    with HipRenderer(cfg) as r:
+        if launch_opts:
+            r.set_launch_opts(*launch_opts)
+        hdr = r.alloc_hdr()
+        for k in range(steps):
+            r.trace(cam, hdr=hdr, with_class=False)
+            st = r.stats()
+            print(f"[bench] trace {k}: {st['kernel_event_ms']:.3f} ms, {st['attempts']} attempts, "
+                  f"{st['rhs_evals']} rhs, {st['accepted']} accepted, max {st['max_attempts_ray']} per ray",
+                  flush=True)
+        torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,7 +451,14 @@ def main():
                     help="only run a plain device copy of this many MiB three times and exit: the "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters (a copy of N MiB reads "
                          "and writes N MiB)")
+    ap.add_argument("--trace-only", type=str, default=None,
+                    help="only trace one workload --steps times with --launch-opts and print its counters "
+                         "(for profiler runs): preset[:tolerance[:fp64|mixed[:spin]]]")
     args = ap.parse_args()
+    if args.trace_only:
+        opts = [int(v) for v in args.launch_opts.split(",")] if args.launch_opts else None
+        trace_only(args.trace_only, args.steps, opts)
+        return
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)
         return

@@ -55,9 +55,10 @@ enum {
  * (config.yaml:40, src/renderer/init.rs:114); both modes here are new. */
 enum {
     KT_PRECISION_FP64 = 0,  /* every operation of the ray in FP64 */
-    KT_PRECISION_MIXED = 1  /* RK stage bulk in FP32; r-solve, error norm,
-                               step controller, disk root, state and
-                               affine accumulators in FP64 */
+    KT_PRECISION_MIXED = 1  /* right-hand-side bulk in FP32; r-solve, stage
+                               algebra, error norm, step controller, disk
+                               root, state and affine accumulators in FP64
+                               (renders below mixed_min_tolerance: FP64) */
 };
 
 /* Equatorial-crossing root (cuda/kerr_trace.cuh:138-174). */
@@ -146,6 +147,11 @@ typedef struct kt_config {
     int32_t precision;                 /* ext: KT_PRECISION_* */
     int32_t rhs_mode;                  /* ext: KT_RHS_* */
     int32_t reserved2;
+    double mixed_min_tolerance;        /* ext, KT_PRECISION_MIXED: a render whose
+                                          tolerance is below this traces in FP64
+                                          (FP32 rounding of the right-hand side
+                                          would reach the error tolerance);
+                                          default 5e-8 */
 } kt_config;
 
 /* Pinhole camera in FP64.  The reference passes 12 f32 scalars plus

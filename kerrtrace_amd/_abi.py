@@ -75,6 +75,7 @@ class KtConfig(C.Structure):
         ("precision", C.c_int32),
         ("rhs_mode", C.c_int32),
         ("reserved2", C.c_int32),
+        ("mixed_min_tolerance", C.c_double),
     ]
 
     def copy(self):

@@ -69,6 +69,11 @@ def parse_float(value, name="value"):
     raise ConfigError(f"{name}: a number is required")
 
 
+# Mixed precision: renders whose tolerance is below this trace in FP64 (FP32
+# rounding of the right-hand side would reach the error tolerance).
+MIXED_MIN_TOLERANCE_DEFAULT = 5e-8
+
+
 def default_config():
     """The reference's config.yaml defaults (config.yaml:1-67)."""
     c = KtConfig()
@@ -105,6 +110,7 @@ def default_config():
     c.disk_root_mode = KT_DISK_ROOT_SECANT_REF
     c.precision = KT_PRECISION_FP64
     c.rhs_mode = KT_RHS_REFERENCE
+    c.mixed_min_tolerance = MIXED_MIN_TOLERANCE_DEFAULT
     return c
 
 
@@ -238,6 +244,7 @@ def config_from_dict(tree):
     c.tol_floor = parse_float(ext.get("tol_floor", 0.0), "tol_floor")
     c.disk_peak_temperature = parse_float(ext.get("peak_temperature", 0.0), "peak_temperature")
     c.mdot_scale = parse_float(ext.get("mdot_scale", 1.0), "mdot_scale")
+    c.mixed_min_tolerance = parse_float(ext.get("mixed_min_tolerance", MIXED_MIN_TOLERANCE_DEFAULT), "mixed_min_tolerance")
     obs = ext.get("observer")
     if obs is not None:
         cam = observer_camera(parse_float(obs["r"], "observer.r"),

@@ -184,70 +184,117 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     return d;
 }
 
-// Mixed-precision right-hand side.  The Kerr-Schild radius is solved in
-// FP64 from the FP64 position (u + sqrt(u^2 + 4a^2y^2) cancels for u < 0
-// near the ring); everything after it runs in FP32.  The result is split
-// as dx/dl = p - corr, dp/dl = kp, with corr = 2H(l.p - pt) l: the
-// free-streaming part p stays with the FP64 state and only the
-// gravitational pieces corr and kp (both O(M/r)) carry FP32 rounding.
+// Mixed-precision right-hand side: FP64 state in, FP64 derivative out,
+// FP32 in between.  The Kerr-Schild radius is solved in FP64 from the FP64
+// position (u + sqrt(u^2 + 4a^2y^2) cancels for u < 0 near the ring); the
+// rest of the factored right-hand side of rhs() runs in FP32 (hardware
+// reciprocal and reciprocal square root, fused multiply-adds).  Its result
+// is split as dx/dl = p - corr, dp/dl = kp with corr = 2H(l.p - pt) l: the
+// free-streaming part p keeps its FP64 value, and only the gravitational
+// pieces corr and kp (both O(M/r)) carry FP32 rounding, so the stage
+// algebra and the embedded error estimate built from these derivatives in
+// FP64 see no O(1) FP32 rounding.
 struct G32 {
     float a, aa, M;
     bool exact;
 };
-struct K32 {
-    float cx, cy, cz;    // corr
-    float px, py, pz;    // dp/dl
-};
-__device__ __forceinline__ K32 rhs_mixed(double x, double y, double z, float pxf, float pyf, float pzf, float ptf,
-                                         double aa_d, const G32& g) {
+__device__ __forceinline__ V6 rhs32(const V6& st, float ptf, const Geo& g, const G32& gf) {
+    const double x = st.x, y = st.y, z = st.z;
     // FP64 radius solve
-    double rho2 = fma(x, x, fma(y, y, z * z));
-    double u = rho2 - aa_d;
-    double s_sq = fma(u, u, 4.0 * aa_d * y * y);
-    double s = sqrt_pos(s_sq);
-    double r2d = 0.5 * (u + s);
-    // FP32 bulk, in the factored form of rhs()
+    const double rho2 = fma(x, x, fma(y, y, z * z));
+    const double u = rho2 - g.aa;
+    const double ya = g.aa * y;
+    const double s = sqrt_pos(fma(u, u, 4.0 * ya * y));
+    const double r2d = 0.5 * (u + s);
+    // FP32 bulk, in the factored form of rhs(); each reciprocal is its own
+    // hardware reciprocal (a shared reciprocal of their product, as rhs()
+    // forms them, costs FP32 accuracy the step control notices)
     const float xf = (float)x, yf = (float)y, zf = (float)z;
-    const float a2 = g.aa, a = g.a;
+    const float pxf = (float)st.px, pyf = (float)st.py, pzf = (float)st.pz;
+    const float a = gf.a, a2 = gf.aa, M = gf.M;
     const float uf = (float)u;
     const float inv_s = __builtin_amdgcn_rcpf((float)s);
     const float r2 = (float)r2d;
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
+    const float ya2 = (a2 * yf) * yf;
     const float inv_denom = __builtin_amdgcn_rcpf(r2 + a2);
-    const float Nx = r * xf - a * zf;
-    const float Nz = r * zf + a * xf;
-    const float lx = Nx * inv_denom;
-    const float ly = yf * inv_r;
-    const float lz = Nz * inv_denom;
-    const float r3 = r2 * r;
-    const float ya2 = a2 * yf * yf;
     const float inv_denomH = __builtin_amdgcn_rcpf(fmaf(r2, r2, ya2));
-    const float H = g.M * r3 * inv_denomH;
+    const float lx = fmaf(r, xf, -(a * zf)) * inv_denom;
+    const float ly = yf * inv_r;
+    const float lz = fmaf(r, zf, a * xf) * inv_denom;
+    const float r3 = r2 * r;
+    const float H = (M * r3) * inv_denomH;
     const float Q = fmaf(lx, pxf, lz * pzf);
     const float lp = fmaf(ly, pyf, Q) - ptf;
-    const float c = 2.0f * H * lp;
-    K32 k;
-    k.cx = c * lx;
-    k.cy = c * ly;
-    k.cz = c * lz;
+    const float c = (2.0f * H) * lp;
     const float inv_2r = 0.5f * inv_r;
-    const float w1 = (1.0f + uf * inv_s) * inv_2r;
-    const float w2 = g.exact ? (1.0f + (uf + 2.0f * a2) * inv_s) * inv_2r
-                             : (1.0f + (uf + 4.0f * a2) * (0.5f * inv_s)) * inv_2r;
+    const float w1 = fmaf(uf, inv_s, 1.0f) * inv_2r;
+    const float w2 = gf.exact ? fmaf(uf + 2.0f * a2, inv_s, 1.0f) * inv_2r
+                              : fmaf(uf + 4.0f * a2, 0.5f * inv_s, 1.0f) * inv_2r;
     const float XP = fmaf(xf, pxf, zf * pzf);
     const float inv_r2 = inv_r * inv_r;
-    const float G = fmaf(-2.0f * r, Q, XP) * inv_denom - yf * pyf * inv_r2;
+    const float G = fmaf(fmaf(-2.0f * r, Q, XP), inv_denom, -((yf * pyf) * inv_r2));
     const float inv_denomH2 = inv_denomH * inv_denomH;
-    const float HR = g.M * r2 * fmaf(3.0f, ya2, -(r2 * r2)) * inv_denomH2;
-    const float HY = -2.0f * g.M * a2 * yf * r3 * inv_denomH2;
+    const float HR = ((M * r2) * inv_denomH2) * fmaf(3.0f, ya2, -(r2 * r2));
+    const float HY = -((2.0f * M) * (a2 * yf)) * (r3 * inv_denomH2);
     const float lp2 = lp * lp;
     const float Z = fmaf(HR, lp2, c * G);
     const float cD = c * inv_denom;
-    k.px = fmaf(xf * w1, Z, cD * fmaf(r, pxf, a * pzf));
-    k.py = fmaf(yf * w2, Z, fmaf(HY, lp2, c * pyf * inv_r));
-    k.pz = fmaf(zf * w1, Z, cD * fmaf(r, pzf, -(a * pxf)));
-    return k;
+    V6 d;
+    d.x = st.px - (double)(c * lx);
+    d.y = st.py - (double)(c * ly);
+    d.z = st.pz - (double)(c * lz);
+    d.px = (double)fmaf(xf * w1, Z, cD * fmaf(r, pxf, a * pzf));
+    d.py = (double)fmaf(yf * w2, Z, fmaf(HY, lp2, (c * pyf) * inv_r));
+    d.pz = (double)fmaf(zf * w1, Z, cD * fmaf(r, pzf, -(a * pxf)));
+    return d;
+}
+
+// Closeness of a ray to the critical curve (the photon-orbit boundary of
+// the shadow), from its flat-space constants of motion at the camera (M
+// units, E = 1): J = x cross n, lambda = J_y, eta = |J|^2 - J_y^2 -
+// a^2 cos^2(theta_cam).  The Boyer-Lindquist radial potential
+//   R(r) = (r^2 + a^2 - a lambda)^2 - (r^2 - 2r + a^2)(eta + (lambda - a)^2)
+// has a double root exactly on the critical curve; near it R has a local
+// minimum close to zero between r_+ and the photon region, and the ray
+// orbits ~ -log|R(r*)| times.  R is sampled on a quadratic grid in
+// [r_+, 12] (dense near the horizon, where the prograde orbits of a rapidly
+// spinning hole sit), the lowest sample is refined with Newton steps on
+// R' = 0, and the key is -log(|R(r*)| / r*^4): large for rays that orbit
+// long.  x: camera position, n: unit direction, a_m = a / M, inv_M = 1/M.
+KT_HDI double critical_key(const double x[3], const double n[3], double a_m, double inv_M) {
+    const double jx = x[1] * n[2] - x[2] * n[1], jy = x[2] * n[0] - x[0] * n[2], jz = x[0] * n[1] - x[1] * n[0];
+    const double rc2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    const double a = a_m;
+    const double cos2 = x[1] * x[1] / rc2;
+    const double lam = jy * inv_M;
+    const double eta = (jx * jx + jz * jz) * (inv_M * inv_M) - a * a * cos2;
+    const double rp = 1.0 + sqrt(fmax(1.0 - a * a, 0.0));
+    const double c2 = eta + (lam - a) * (lam - a);
+    const double r_lo = rp * 1.0001, span = 12.0 - rp;
+    double best = 3.0e38, r_best = r_lo;
+    const int NS = 32;
+    for (int i = 0; i < NS; ++i) {
+        const double t = (double)i / (double)(NS - 1);
+        const double r = r_lo + span * t * t;
+        const double A = r * r + a * a - a * lam;
+        const double R = A * A - (r * r - 2.0 * r + a * a) * c2;
+        if (R < best) {
+            best = R;
+            r_best = r;
+        }
+    }
+    double r = r_best;
+    for (int it = 0; it < 3; ++it) {
+        const double A = r * r + a * a - a * lam;
+        const double d1 = 4.0 * r * A - (2.0 * r - 2.0) * c2;
+        const double d2 = 4.0 * A + 8.0 * r * r - 2.0 * c2;
+        if (d2 > 0.0) r = fmin(fmax(r - d1 / d2, r_lo), 12.0);
+    }
+    const double A = r * r + a * a - a * lam;
+    const double R = fmin(A * A - (r * r - 2.0 * r + a * a) * c2, best);
+    return -log(fabs(R) / (r * r * r * r) + 1e-12);
 }
 
 // DOPRI5 continuous extension (cuda/kerr_math.cuh:186-219; Hairer, Norsett

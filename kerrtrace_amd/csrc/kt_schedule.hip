@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include "kt_internal.hpp"
+#include "kt_math.hpp"
 
 namespace kt {
 
@@ -60,38 +61,7 @@ __global__ __launch_bounds__(256) void cost_keys(const RenderParams* __restrict_
         n[0] *= inv;
         n[1] *= inv;
         n[2] *= inv;
-        const double x = P.cam[0], y = P.cam[1], z = P.cam[2];
-        const double jx = y * n[2] - z * n[1], jy = z * n[0] - x * n[2], jz = x * n[1] - y * n[0];
-        const double rc2 = x * x + y * y + z * z;
-        const double a = P.a * P.inv_M;  // in units of M
-        const double cos2 = y * y / rc2;
-        const double lam = jy * P.inv_M;
-        const double eta = (jx * jx + jz * jz) * (P.inv_M * P.inv_M) - a * a * cos2;
-        const double rp = 1.0 + sqrt(fmax(1.0 - a * a, 0.0));
-        const double c2 = eta + (lam - a) * (lam - a);
-        const double r_lo = rp * 1.0001, span = 12.0 - rp;
-        double best = 3.0e38, r_best = r_lo;
-        const int NS = 32;
-        for (int i = 0; i < NS; ++i) {
-            const double t = (double)i / (double)(NS - 1);
-            const double r = r_lo + span * t * t;
-            const double A = r * r + a * a - a * lam;
-            const double R = A * A - (r * r - 2.0 * r + a * a) * c2;
-            if (R < best) {
-                best = R;
-                r_best = r;
-            }
-        }
-        double r = r_best;
-        for (int it = 0; it < 3; ++it) {
-            const double A = r * r + a * a - a * lam;
-            const double d1 = 4.0 * r * A - (2.0 * r - 2.0) * c2;
-            const double d2 = 4.0 * A + 8.0 * r * r - 2.0 * c2;
-            if (d2 > 0.0) r = fmin(fmax(r - d1 / d2, r_lo), 12.0);
-        }
-        const double A = r * r + a * a - a * lam;
-        const double R = fmin(A * A - (r * r - 2.0 * r + a * a) * c2, best);
-        key = (float)(-log(fabs(R) / (r * r * r * r) + 1e-12));
+        key = (float)critical_key(P.cam, n, P.a * P.inv_M, P.inv_M);
     }
     for (int off = 32; off > 0; off >>= 1) key = fmaxf(key, __shfl_xor(key, off));
     if (lane == 0) {

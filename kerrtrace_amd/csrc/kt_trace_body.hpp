@@ -1,0 +1,188 @@
+// The persistent tracer's loop, shared by the FP64 kernel (kt_trace.hip)
+// and the mixed-precision kernel (kt_mixed.hip).
+//
+// The reference (cuda/kernel.cu:88-142) runs one thread per pixel on a
+// 16x16 grid, so every block waits for its slowest photon-ring ray.  Here a
+// grid of resident waves pulls work from one queue instead:
+//
+//  * one lane = one ray; its whole state (position, momentum, FSAL k1,
+//    step size, colour, transmittance) lives in VGPRs for its lifetime;
+//  * the loop body is ONE DP5(4) attempt (six right-hand sides), so lanes
+//    that retry a rejected step and lanes that move on stay converged;
+//  * when a lane's ray terminates it goes idle; once `refill_threshold`
+//    lanes of the wave are idle the wave claims that many new work items
+//    with a single atomic on the queue head and hands them to the idle
+//    lanes (ballot + mbcnt), so terminated rays are compacted out without
+//    moving any state;
+//  * work items are ordered in square pixel tiles so a wave's rays start
+//    coherent; a batched launch concatenates the work items of many renders
+//    and each lane carries its render's hot parameters.
+//
+// Per ray the arithmetic is that of cuda/kerr_trace.cuh:4-240 in binary64
+// (see kt_math.hpp for the reproducibility contract).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_internal.hpp"
+#include "kt_math.hpp"
+#include "kt_tracer_common.hpp"
+
+namespace kt {
+
+// The loop of a persistent tracer kernel (one lane = one ray, one loop
+// turn = one DP5(4) attempt of every active lane).
+// RECORD: the differentiable renderer's forward pass, which also writes
+// the step log (RenderParams::log) the adjoint pass replays.
+// MIXED: the mixed-precision tracer: the right-hand sides run in rhs32
+// (FP64 radius solve, FP32 bulk); the stage algebra, state, error norm,
+// controller, dense output and shading stay FP64.
+template <bool BATCH, bool RECORD, bool MIXED>
+__device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, int n_renders, uint64_t total_items,
+                                           int refill_threshold, DevCounters* __restrict__ counters) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    Lane L;
+    L.active = false;
+    L.finished = false;
+    L.post = false;
+    L.crossing = false;
+    L.rid = 0;
+    L.overflow = false;
+    L.overflow_T = 0.0;
+    L.ray_att = 0;
+    L.chunk = kNoChunk;
+    L.nrec = 0;
+    LaneCounters cnt;
+    bool drained = false;
+    int prio = 0;
+    const uint32_t long_ray = prm<BATCH>(Ps, 0).long_ray;
+    const int isolate = (int)prm<BATCH>(Ps, 0).isolate;
+    Hot H = load_hot(prm<BATCH>(Ps, 0));
+    if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
+
+    for (;;) {
+        // ---- second half of last iteration's commit, then rays that ended:
+        // write the pixel or start its next sample.  Both run here, where no
+        // stage derivative is live.
+        if (L.post) {
+            L.post = false;
+            post_commit(L, prm<BATCH>(Ps, L.rid), H, cnt);
+        }
+        if (L.finished) {
+            L.finished = false;
+            cnt.max_att = max(cnt.max_att, L.ray_att);
+            if (RECORD) record_end(L, prm<BATCH>(Ps, L.rid));
+            end_sample(L, prm<BATCH>(Ps, L.rid), H);
+        }
+        // ---- refill idle lanes from the work queue (wave-uniform branch)
+        // (a wave isolated around a long ray takes no new rays until it ends)
+        if (isolate == 0 || prio < isolate)
+            refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
+
+        if (drained && __ballot(L.active) == 0) break;
+        // a wave holding a long-running ray (photon-ring orbits) issues
+        // ahead of the other wave(s) on its SIMD, so its serial chain of
+        // steps, which bounds the frame time, is not slowed by them
+        wave_priority(L, long_ray, prio);
+        if (!L.active || L.finished) continue;
+
+        // ---- one DP5(4) attempt (cuda/kerr_trace.cuh:83-122)
+        const Geo g = H.g;
+        const V6 s = L.s;
+        const G32 g32{(float)g.a, (float)g.aa, (float)g.M, g.exact};
+        const float ptf = (float)L.pt;
+        const double pt = L.pt;
+#define KT_RHS(st) (MIXED ? rhs32((st), ptf, g, g32) : rhs((st), pt, g))
+        if (!L.have_k1) {
+            L.k1 = KT_RHS(s);
+            L.have_k1 = true;
+            cnt.rhs++;
+        }
+        const double hs = L.h;
+        V6 ns, eacc, dacc;
+        const V6 k1s = L.k1;
+        {
+            const V6 k1 = L.k1;
+            // stage inputs s + h * sum_j a_ij k_j as fma chains
+#define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
+#define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
+#define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
+#define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
+#define KT_ST6(c) \
+    fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
+#define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
+            const V6 k2 = KT_RHS(KT_STAGE(KT_ST2));
+            const V6 k3 = KT_RHS(KT_STAGE(KT_ST3));
+            const V6 k4 = KT_RHS(KT_STAGE(KT_ST4));
+            const V6 k5 = KT_RHS(KT_STAGE(KT_ST5));
+            const V6 in6 = KT_STAGE(KT_ST6);
+            // The 5th-order solution b.k and the error (b - bhat).k are fma
+            // chains in stage order; their leading terms are folded now so
+            // k2..k5 are dead while stage 6 and 7 are evaluated.
+#define KT_B4ACC(c) fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))
+#define KT_E5ACC(c) fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))
+            V6 bacc = KT_STAGE(KT_B4ACC);
+            eacc = KT_STAGE(KT_E5ACC);
+            // dense-output sum d1 k1 + d3 k3 + ... + d7 k7, left to right
+#define KT_D5ACC(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c)
+            dacc = KT_STAGE(KT_D5ACC);
+            const V6 k6 = KT_RHS(in6);
+#define KT_NS(c) fma(hs, fma(KT_B6, k6.c, bacc.c), s.c)
+#define KT_EACC6(c) fma(KT_E6, k6.c, eacc.c)
+            ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
+            eacc = KT_STAGE(KT_EACC6);
+#define KT_D6ACC(c) (dacc.c + Dense::d6 * k6.c)
+            dacc = KT_STAGE(KT_D6ACC);
+#undef KT_ST2
+#undef KT_ST3
+#undef KT_ST4
+#undef KT_ST5
+#undef KT_ST6
+#undef KT_B4ACC
+#undef KT_E5ACC
+#undef KT_D5ACC
+#undef KT_D6ACC
+#undef KT_NS
+#undef KT_EACC6
+        }
+        const V6 k7 = KT_RHS(ns);
+#undef KT_RHS
+        cnt.rhs += 6;
+#define KT_D7ACC(c) (dacc.c + Dense::d7 * k7.c)
+        dacc = KT_STAGE(KT_D7ACC);
+#undef KT_D7ACC
+        // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
+        const double tol = L.tol;
+#define KT_ERR(c) div_pos(fabs(hs * fma(KT_E7, k7.c, eacc.c)), tol * fmax(fabs(s.c), 1.0))
+        double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
+        double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
+#undef KT_ERR
+#undef KT_STAGE
+        double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
+        if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
+            // a non-finite step error: the library quotients (special values)
+#define KT_ERRX(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
+            err = fmax(fmax(fmax(KT_ERRX(x), KT_ERRX(y)), fmax(KT_ERRX(z), KT_ERRX(px))),
+                       fmax(KT_ERRX(py), KT_ERRX(pz)));
+#undef KT_ERRX
+        }
+        cnt.attempts++;
+        L.ray_att++;
+        L.att_in_step++;
+        const bool accepted = err <= 1.0;
+        // controller without clamps (kerr_trace.cuh:119)
+        L.h = hs * 0.9 * pow_m02(err);
+        if (!accepted && L.att_in_step < H.max_attempts) continue;  // retry from the same state
+
+        // ---- commit; a rejected last attempt is committed too (kerr_trace.cuh:123-134)
+        if (accepted) cnt.accepted++;
+        if (accepted) L.k1 = k7;  // FSAL
+        L.have_k1 = accepted;
+        if (RECORD) record_step(L, prm<BATCH>(Ps, L.rid), s, hs, lane);
+        commit(L, s, ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
+    }
+
+    flush_counters(cnt, counters, lane);
+}
+
+}  // namespace kt

