@@ -365,6 +365,29 @@ def grad_leg(args):
                                       "backward_ms": {str(k): round(v, 4) for k, v in times.items()},
                                       "backward_over_forward": round(times[best_k] / tf, 4)}
         out["backward_over_plain_forward"] = round(tb / out["plain_forward_ms"], 4)
+        # the other precision pairs: FP64 forward + FP32 reversal, and the
+        # workload's stated mixed forward + FP32 adjoint (FP64 crossing and
+        # ray-start derivatives in both)
+        from kerrtrace_amd._abi import KT_ADJOINT_FP32
+        out["precision_modes"] = {}
+        for name, prec in (("fp64_forward_fp32_adjoint", K.KT_PRECISION_FP64),
+                           ("mixed_forward_fp32_adjoint", K.KT_PRECISION_MIXED)):
+            cm = cfg.copy()
+            cm.precision = prec
+            with GradRenderer(cm, checkpoint_interval=32, adjoint_precision=KT_ADJOINT_FP32) as gm:
+                fw, bw = [], []
+                for it in range(args.warmup + args.steps):
+                    hdr, _ = gm.forward(K.GRAD_START)
+                    dl = 2.0 * (hdr - target)
+                    dl[..., 3] = 0.0
+                    gm.backward(dl)
+                    stm = gm.stats()
+                    if it >= args.warmup:
+                        fw.append(stm["forward_ms"])
+                        bw.append(stm["backward_ms"])
+                tfm, tbm = sum(fw) / len(fw), sum(bw) / len(bw)
+                out["precision_modes"][name] = {"forward_ms": round(tfm, 4), "backward_ms": round(tbm, 4),
+                                                "backward_over_forward": round(tbm / tfm, 4)}
         if args.adam_iters > 0:
             torch.cuda.synchronize()
             t0 = time.perf_counter()

@@ -315,7 +315,8 @@ kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream,
  * observer distance r_obs [M], accretion-rate scale mdot}: camera at
  * r_obs (sin i, cos i, 0) looking at the hole (rgt = (0, 0, -1)), vertical
  * FOV opts->fov_deg, escape radius = r_obs, mass and every other knob from
- * cfg (FP64 precision, exact right-hand side and spp = 1 are required).
+ * cfg (exact right-hand side and spp = 1 are required; cfg->precision
+ * chooses the forward tracer, FP64 or mixed).
  * kt_grad_forward traces the frame with the FP64 tracer and logs every
  * ray's committed steps (a state checkpoint every checkpoint_interval
  * steps); kt_grad_backward returns, for the frame of the last forward,
@@ -324,10 +325,19 @@ kt_status kt_batch_autotune(kt_batch* b, float* const* hdr, void* stream,
  * table cell) held fixed.  grad: device double[4]; grad_px: NULL or device
  * double[W*H*4] per-pixel contributions; dl_dhdr: device float4[W*H]
  * (alpha ignored).  Errors are read with kt_last_error(NULL, ...). */
+enum {
+    KT_ADJOINT_FP64 = 0,         /* every operation of the reverse pass in FP64 */
+    KT_ADJOINT_FP32 = 1          /* step reversals (right-hand-side VJPs, stage
+                                    and state adjoints) in FP32; re-integration,
+                                    disk-crossing (dense-output root) and ray
+                                    start derivatives in FP64 */
+};
 typedef struct kt_grad_opts {
     double fov_deg;              /* vertical field of view */
     int32_t checkpoint_interval; /* committed steps per checkpoint, 1..256 */
     int32_t log_steps_per_ray;   /* step-log room: W*H*(1 + ceil(this / interval)) chunks */
+    int32_t adjoint_precision;   /* KT_ADJOINT_* */
+    int32_t reserved;
 } kt_grad_opts;
 typedef struct kt_grad_stats {
     double forward_ms;           /* HIP events around the recording trace */

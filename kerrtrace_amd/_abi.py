@@ -14,6 +14,8 @@ KT_ERR_LUT_OVERFLOW = -4
 KT_ERR_QUEUE_ORDER = -5
 KT_ERR_INVALID_ARGUMENT = -6
 KT_ERR_CAPACITY = -7
+KT_ADJOINT_FP64 = 0
+KT_ADJOINT_FP32 = 1
 
 KT_PRECISION_FP64 = 0
 KT_PRECISION_MIXED = 1
@@ -157,6 +159,8 @@ class KtGradOpts(C.Structure):
         ("fov_deg", C.c_double),
         ("checkpoint_interval", C.c_int32),
         ("log_steps_per_ray", C.c_int32),
+        ("adjoint_precision", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -202,6 +202,17 @@ struct StepOut {
 #define KTA_B5 (-2187.0 / 6784.0)
 #define KTA_B6 (11.0 / 84.0)
 
+// The forward pass's right-hand side: rhs() (FP64 tracer) or rhs32 (mixed
+// tracer, device only).
+template <bool MIXED>
+KT_HDI V6 rhs_fwd(const V6& st, double pt, const Geo& g) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (MIXED) return rhs32(st, (float)pt, g, G32{(float)g.a, (float)g.aa, (float)g.M, g.exact});
+#endif
+    return rhs(st, pt, g);
+}
+
+template <bool MIXED = false>
 KT_HDI StepOut dp_step(const V6& s, double hs, double pt, const Geo& g, V6* Y) {
 #define KTA_ST2(c) fma(hs, KTA_A21 * k1.c, s.c)
 #define KTA_ST3(c) fma(hs, fma(KTA_A32, k2.c, KTA_A31 * k1.c), s.c)
@@ -211,21 +222,21 @@ KT_HDI StepOut dp_step(const V6& s, double hs, double pt, const Geo& g, V6* Y) {
     fma(hs, fma(KTA_A65, k5.c, fma(KTA_A64, k4.c, fma(KTA_A63, k3.c, fma(KTA_A62, k2.c, KTA_A61 * k1.c)))), s.c)
 #define KTA_V(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
     StepOut o;
-    const V6 k1 = rhs(s, pt, g);
+    const V6 k1 = rhs_fwd<MIXED>(s, pt, g);
     const V6 y2 = KTA_V(KTA_ST2);
-    const V6 k2 = rhs(y2, pt, g);
+    const V6 k2 = rhs_fwd<MIXED>(y2, pt, g);
     const V6 y3 = KTA_V(KTA_ST3);
-    const V6 k3 = rhs(y3, pt, g);
+    const V6 k3 = rhs_fwd<MIXED>(y3, pt, g);
     const V6 y4 = KTA_V(KTA_ST4);
-    const V6 k4 = rhs(y4, pt, g);
+    const V6 k4 = rhs_fwd<MIXED>(y4, pt, g);
     const V6 y5 = KTA_V(KTA_ST5);
-    const V6 k5 = rhs(y5, pt, g);
+    const V6 k5 = rhs_fwd<MIXED>(y5, pt, g);
     const V6 y6 = KTA_V(KTA_ST6);
 #define KTA_B4ACC(c) fma(KTA_B5, k5.c, fma(KTA_B4, k4.c, fma(KTA_B3, k3.c, KTA_B1 * k1.c)))
 #define KTA_D5ACC(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c)
     const V6 bacc = KTA_V(KTA_B4ACC);
     const V6 dacc = KTA_V(KTA_D5ACC);
-    const V6 k6 = rhs(y6, pt, g);
+    const V6 k6 = rhs_fwd<MIXED>(y6, pt, g);
 #define KTA_NS(c) fma(hs, fma(KTA_B6, k6.c, bacc.c), s.c)
 #define KTA_D6ACC(c) (dacc.c + Dense::d6 * k6.c)
     o.ns = KTA_V(KTA_NS);
@@ -311,6 +322,190 @@ KT_HDI V6 dp_step_reverse(const V6& s, const V6& ns, const V6* Y, double hs, dou
     v6_axpy(l1, hs * KTA_A21, lY);
     // stage 1 at s
     rhs_vjp(s, pt, g, l1, ls, lam_pt, lam_a);
+    return ls;
+}
+
+// ------------------------------------------------ FP32 adjoint variant
+// The step reversal with its right-hand-side VJPs, stage adjoints and state
+// adjoint in FP32 (the forward states and stage inputs come from the FP64
+// re-integration and are rounded once): half the registers of the FP64
+// reversal.  Only the VJPs' evaluation points and the adjoint recursion
+// lose digits; the crossing adjoint (dense-output root, emission) stays FP64.
+struct V6f {
+    float x, y, z, px, py, pz;
+};
+KT_HDI V6f v6f_zero() { return V6f{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+KT_HDI V6f to_f(const V6& v) {
+    return V6f{(float)v.x, (float)v.y, (float)v.z, (float)v.px, (float)v.py, (float)v.pz};
+}
+KT_HDI V6 to_d(const V6f& v) {
+    return V6{(double)v.x, (double)v.y, (double)v.z, (double)v.px, (double)v.py, (double)v.pz};
+}
+KT_HDI void v6f_axpy(V6f& acc, float a, const V6f& x) {
+    acc.x = fmaf(a, x.x, acc.x);
+    acc.y = fmaf(a, x.y, acc.y);
+    acc.z = fmaf(a, x.z, acc.z);
+    acc.px = fmaf(a, x.px, acc.px);
+    acc.py = fmaf(a, x.py, acc.py);
+    acc.pz = fmaf(a, x.pz, acc.pz);
+}
+KT_HDI void v6f_add(V6f& acc, const V6f& x) {
+    acc.x += x.x;
+    acc.y += x.y;
+    acc.z += x.z;
+    acc.px += x.px;
+    acc.py += x.py;
+    acc.pz += x.pz;
+}
+// FP32 dual number
+struct F1 {
+    float v, d;
+};
+KT_HDI F1 operator+(F1 a, F1 b) { return F1{a.v + b.v, a.d + b.d}; }
+KT_HDI F1 operator-(F1 a, F1 b) { return F1{a.v - b.v, a.d - b.d}; }
+KT_HDI F1 operator-(F1 a) { return F1{-a.v, -a.d}; }
+KT_HDI F1 operator*(F1 a, F1 b) { return F1{a.v * b.v, fmaf(a.d, b.v, a.v * b.d)}; }
+KT_HDI F1 operator*(float a, F1 b) { return F1{a * b.v, a * b.d}; }
+KT_HDI F1 operator+(F1 a, float b) { return F1{a.v + b, a.d}; }
+KT_HDI F1 operator-(F1 a, float b) { return F1{a.v - b, a.d}; }
+KT_HDI F1 ffma(F1 a, F1 b, F1 c) { return F1{fmaf(a.v, b.v, c.v), fmaf(a.d, b.v, fmaf(a.v, b.d, c.d))}; }
+KT_HDI float rcp32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+KT_HDI float rsq32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+KT_HDI F1 frcp(F1 b) {
+    const float iv = rcp32(b.v);
+    return F1{iv, -(b.d * iv) * iv};
+}
+KT_HDI F1 fsqrt(F1 a) {
+    const float rs = rsq32(a.v);
+    return F1{a.v * rs, a.d * (0.5f * rs)};
+}
+
+// rhs_vjp in FP32 (the same dual pass; see rhs_vjp).
+KT_HDI void rhs_vjp32(const V6f& Y, float pt, float a, float a2, float M, const V6f& w, V6f& lamY, float& lam_pt,
+                      float& lam_a) {
+    const F1 x{Y.x, -w.px}, y{Y.y, -w.py}, z{Y.z, -w.pz};
+    const F1 px{Y.px, w.x}, py{Y.py, w.y}, pz{Y.pz, w.z};
+    const F1 rho2 = ffma(x, x, ffma(y, y, z * z));
+    const F1 u = rho2 - a2;
+    const F1 ya = a2 * y;
+    const F1 yy = y * y;
+    const F1 s = fsqrt(ffma(u, u, 4.0f * (ya * y)));
+    const F1 r2 = 0.5f * (u + s);
+    const F1 r = fsqrt(r2);
+    const F1 denom = r2 + a2;
+    const F1 denomH = ffma(r2, r2, ya * y);
+    const F1 inv_s = frcp(s);
+    const F1 inv_r = frcp(r);
+    const F1 inv_denom = frcp(denom);
+    const F1 inv_denomH = frcp(denomH);
+    const F1 Nx = ffma(r, x, -(a * z));
+    const F1 Nz = ffma(r, z, a * x);
+    const F1 lx = Nx * inv_denom, ly = y * inv_r, lz = Nz * inv_denom;
+    const F1 r3 = r2 * r;
+    const F1 H = (M * r3) * inv_denomH;
+    const F1 Q = ffma(lx, px, lz * pz);
+    const F1 lp = ffma(ly, py, Q) - pt;
+    const F1 c = (2.0f * H) * lp;
+    const F1 fx = px - c * lx, fy = py - c * ly, fz = pz - c * lz;
+    const F1 inv_2r = 0.5f * inv_r;
+    const F1 w1 = (u * inv_s + 1.0f) * inv_2r;
+    const F1 w2 = ((u + 2.0f * a2) * inv_s + 1.0f) * inv_2r;
+    const F1 XP = ffma(x, px, z * pz);
+    const F1 inv_r2 = inv_r * inv_r;
+    const F1 G = ffma(-2.0f * r, Q, XP) * inv_denom - (y * py) * inv_r2;
+    const F1 inv_denomH2 = inv_denomH * inv_denomH;
+    const F1 HR = ((M * r2) * inv_denomH2) * (3.0f * (ya * y) - r2 * r2);
+    const F1 HY = -(2.0f * M) * ya * (r3 * inv_denomH2);
+    const F1 lp2 = lp * lp;
+    const F1 Z = ffma(HR, lp2, c * G);
+    const F1 cD = c * inv_denom;
+    const F1 fpx = ffma(x * w1, Z, cD * ffma(r, px, a * pz));
+    const F1 fpy = ffma(y * w2, Z, ffma(HY, lp2, (c * py) * inv_r));
+    const F1 fpz = ffma(z * w1, Z, cD * ffma(r, pz, -(a * px)));
+    lamY.x -= fpx.d;
+    lamY.y -= fpy.d;
+    lamY.z -= fpz.d;
+    lamY.px += fx.d;
+    lamY.py += fy.d;
+    lamY.pz += fz.d;
+    lam_pt += 2.0f * fmaf(H.d, lp.v, H.v * lp.d);
+    const F1 r2_a = a * ((2.0f * yy - u) * inv_s - 1.0f);
+    const F1 r_a = r2_a * inv_2r;
+    const F1 D_a = r2_a + 2.0f * a;
+    const F1 inv_denom2 = inv_denom * inv_denom;
+    const F1 lx_a = ((r_a * x - z) * denom - Nx * D_a) * inv_denom2;
+    const F1 lz_a = ((r_a * z + x) * denom - Nz * D_a) * inv_denom2;
+    const F1 ly_a = -(y * r_a) * inv_r2;
+    const F1 DH_a = 2.0f * r2 * r2_a + (2.0f * a) * yy;
+    const F1 H_a = (M * inv_denomH2) * ((3.0f * r2) * denomH * r_a - r3 * DH_a);
+    const F1 lp_a = ffma(lx_a, px, ffma(ly_a, py, lz_a * pz));
+    const F1 ham_a = -(H_a * lp2 + 2.0f * H * lp * lp_a);
+    lam_a += ham_a.d;
+}
+
+// dp_step_reverse in FP32 (same recursion; see dp_step_reverse).
+KT_HDI V6f dp_step_reverse32(const V6f& s, const V6f& ns, const V6f* Y, float hs, float pt, float a, float a2,
+                             float M, V6f lam_ns, const V6f* lk, const V6f& lam_s_dir, float& lam_pt, float& lam_a) {
+    if (lk) rhs_vjp32(ns, pt, a, a2, M, lk[5], lam_ns, lam_pt, lam_a);  // k7 = f(ns)
+    V6f l1 = v6f_zero(), l2 = v6f_zero(), l3 = v6f_zero(), l4 = v6f_zero(), l5 = v6f_zero(), l6 = v6f_zero();
+    v6f_axpy(l1, hs * (float)KTA_B1, lam_ns);
+    v6f_axpy(l3, hs * (float)KTA_B3, lam_ns);
+    v6f_axpy(l4, hs * (float)KTA_B4, lam_ns);
+    v6f_axpy(l5, hs * (float)KTA_B5, lam_ns);
+    v6f_axpy(l6, hs * (float)KTA_B6, lam_ns);
+    if (lk) {
+        v6f_add(l1, lk[0]);
+        v6f_add(l3, lk[1]);
+        v6f_add(l4, lk[2]);
+        v6f_add(l5, lk[3]);
+        v6f_add(l6, lk[4]);
+    }
+    V6f ls = lam_ns;
+    v6f_add(ls, lam_s_dir);
+    V6f lY;
+    lY = v6f_zero();
+    rhs_vjp32(Y[4], pt, a, a2, M, l6, lY, lam_pt, lam_a);
+    v6f_add(ls, lY);
+    v6f_axpy(l1, hs * (float)KTA_A61, lY);
+    v6f_axpy(l2, hs * (float)KTA_A62, lY);
+    v6f_axpy(l3, hs * (float)KTA_A63, lY);
+    v6f_axpy(l4, hs * (float)KTA_A64, lY);
+    v6f_axpy(l5, hs * (float)KTA_A65, lY);
+    lY = v6f_zero();
+    rhs_vjp32(Y[3], pt, a, a2, M, l5, lY, lam_pt, lam_a);
+    v6f_add(ls, lY);
+    v6f_axpy(l1, hs * (float)KTA_A51, lY);
+    v6f_axpy(l2, hs * (float)KTA_A52, lY);
+    v6f_axpy(l3, hs * (float)KTA_A53, lY);
+    v6f_axpy(l4, hs * (float)KTA_A54, lY);
+    lY = v6f_zero();
+    rhs_vjp32(Y[2], pt, a, a2, M, l4, lY, lam_pt, lam_a);
+    v6f_add(ls, lY);
+    v6f_axpy(l1, hs * (float)KTA_A41, lY);
+    v6f_axpy(l2, hs * (float)KTA_A42, lY);
+    v6f_axpy(l3, hs * (float)KTA_A43, lY);
+    lY = v6f_zero();
+    rhs_vjp32(Y[1], pt, a, a2, M, l3, lY, lam_pt, lam_a);
+    v6f_add(ls, lY);
+    v6f_axpy(l1, hs * (float)KTA_A31, lY);
+    v6f_axpy(l2, hs * (float)KTA_A32, lY);
+    lY = v6f_zero();
+    rhs_vjp32(Y[0], pt, a, a2, M, l2, lY, lam_pt, lam_a);
+    v6f_add(ls, lY);
+    v6f_axpy(l1, hs * (float)KTA_A21, lY);
+    rhs_vjp32(s, pt, a, a2, M, l1, ls, lam_pt, lam_a);
     return ls;
 }
 

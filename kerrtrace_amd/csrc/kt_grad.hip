@@ -61,6 +61,7 @@ struct CrossAdj {
     double lamTr, lam_pt, lam_kt, g_a, g_m;
     bool lit;
 };
+template <bool MIXED>
 static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& P, const GradConsts& G, V6 sn, V6 ns,
                                                          V6 y3, V6 y4, V6 y5, V6 y6, double hs, double pt,
                                                          double kt, double Tr_n, double W0, double W1, double W2,
@@ -69,8 +70,8 @@ static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& 
     o.lit = false;
     const Geo g{P.a, P.aa, P.M, true};
     // the step's dense-output ingredients, as the forward pass formed them
-    const V6 k1 = rhs(sn, pt, g), k3 = rhs(y3, pt, g), k4 = rhs(y4, pt, g);
-    const V6 k5 = rhs(y5, pt, g), k6 = rhs(y6, pt, g), k7 = rhs(ns, pt, g);
+    const V6 k1 = rhs_fwd<MIXED>(sn, pt, g), k3 = rhs_fwd<MIXED>(y3, pt, g), k4 = rhs_fwd<MIXED>(y4, pt, g);
+    const V6 k5 = rhs_fwd<MIXED>(y5, pt, g), k6 = rhs_fwd<MIXED>(y6, pt, g), k7 = rhs_fwd<MIXED>(ns, pt, g);
 #define KT_DS(c) ((Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c) + Dense::d6 * k6.c) + \
                      Dense::d7 * k7.c
     const V6 dsum{KT_DS(x), KT_DS(y), KT_DS(z), KT_DS(px), KT_DS(py), KT_DS(pz)};
@@ -95,8 +96,13 @@ static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& 
 }
 
 // One pixel's gradient contribution: out = d(sum_c w_c hdr_c)/d theta.
-__device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, int py, const double w[3],
-                            const LaneScratch& sc, double out[4]) {
+// MIXED: the forward pass was the mixed tracer (rhs32), whose arithmetic
+// the re-integration repeats.  F32: the step reversals (right-hand-side
+// VJPs, stage and state adjoints) in FP32; the re-integration, the crossing
+// adjoint (dense-output root, emission) and the ray start stay FP64.
+template <bool MIXED, bool F32>
+__device__ __forceinline__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, int py,
+                                            const double w[3], const LaneScratch& sc, double out[4]) {
     out[0] = out[1] = out[2] = out[3] = 0.0;
     const GradLog& LG = P.log;
     const size_t pix = (size_t)py * (size_t)P.W + (size_t)px;
@@ -112,6 +118,9 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
     double lamTr = ((cls & 3) == 2 && (cls & 8)) ? P.sky_int * (W0 + W1 + W2) : 0.0;
     V6 lam = v6_zero();
     double lam_pt = 0.0, lam_kt = 0.0, g_a = 0.0, g_m = 0.0;
+    V6f lamf = v6f_zero();  // F32: the state adjoint
+    float lam_ptf = 0.f, g_af = 0.f;
+    const float af = (float)P.a, a2f = (float)P.aa, Mf = (float)P.M, ptf = (float)pt;
     uint32_t chunk = LG.head[pix];
     if (chunk >= LG.capacity) return;
     uint32_t seg_start = ((N - 1) / K) * K;
@@ -123,7 +132,7 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
         V6 s{ch[0], ch[1], ch[2], ch[3], ch[4], ch[5]};
         for (uint32_t u = 0; u < len; ++u) {
             sc.put6(6 * u, s);
-            s = dp_step(s, ch[8 + u], pt, g, nullptr).ns;
+            s = dp_step<MIXED>(s, ch[8 + u], pt, g, nullptr).ns;
         }
         sc.put6(j_end, s);
         for (int u = (int)len - 1; u >= 0; --u) {
@@ -133,7 +142,7 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
             const double Tr_n = ch[8 + K + u];
             // the step's stage inputs, formed as the forward pass formed them
             V6 Y[5];
-            (void)dp_step(sn, hs, pt, g, Y);
+            (void)dp_step<MIXED>(sn, hs, pt, g, Y);
             const double r_now = ks_r(ns.x, ns.y, ns.z, P.aa);
             const bool horizon = r_now < P.rh_eps;
             const bool crossing = !horizon && ((sn.y < 0.0 && ns.y >= 0.0) || (sn.y > 0.0 && ns.y <= 0.0));
@@ -141,8 +150,8 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
             V6 lam_s_dir = v6_zero();
             bool have_lk = false;
             if (crossing) {
-                CrossAdj ca = crossing_adjoint(P, G, sn, ns, Y[1], Y[2], Y[3], Y[4], hs, pt, kt, Tr_n, W0, W1, W2,
-                                               lamTr);
+                CrossAdj ca = crossing_adjoint<MIXED>(P, G, sn, ns, Y[1], Y[2], Y[3], Y[4], hs, pt, kt, Tr_n, W0, W1,
+                                                      W2, lamTr);
                 if (ca.lit) {
                     lamTr = ca.lamTr;
                     lam_pt += ca.lam_pt;
@@ -151,11 +160,24 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
                     g_m += ca.g_m;
                     for (int i = 0; i < 6; ++i) lk[i] = ca.lk[i];
                     lam_s_dir = ca.lam_s;
-                    v6_add(lam, ca.lam_ns);
+                    if (F32) {
+                        v6f_add(lamf, to_f(ca.lam_ns));
+                    } else {
+                        v6_add(lam, ca.lam_ns);
+                    }
                     have_lk = true;
                 }
             }
-            lam = dp_step_reverse(sn, ns, Y, hs, pt, g, lam, have_lk ? lk : nullptr, lam_s_dir, lam_pt, g_a);
+            if (F32) {
+                V6f Yf[5], lkf[6];
+                for (int i = 0; i < 5; ++i) Yf[i] = to_f(Y[i]);
+                if (have_lk)
+                    for (int i = 0; i < 6; ++i) lkf[i] = to_f(lk[i]);
+                lamf = dp_step_reverse32(to_f(sn), to_f(ns), Yf, (float)hs, ptf, af, a2f, Mf, lamf,
+                                         have_lk ? lkf : nullptr, to_f(lam_s_dir), lam_ptf, g_af);
+            } else {
+                lam = dp_step_reverse(sn, ns, Y, hs, pt, g, lam, have_lk ? lk : nullptr, lam_s_dir, lam_pt, g_a);
+            }
         }
         if (seg_start == 0) break;
         chunk = (uint32_t)__double_as_longlong(ch[6]);
@@ -164,6 +186,11 @@ __device__ void ray_adjoint(const RenderParams& P, const GradConsts& G, int px, 
             return;
         }
         seg_start -= K;
+    }
+    if (F32) {
+        lam = to_d(lamf);
+        lam_pt += (double)lam_ptf;
+        g_a += (double)g_af;
     }
     // ray start: s0, pt, kt as functions of spin, inclination, distance
     for (int dir = 0; dir < 3; ++dir) {
@@ -193,6 +220,7 @@ __global__ __launch_bounds__(256) void grad_keys(const uint32_t* __restrict__ ns
     if (lane == 0) keys[tile] = (uint8_t)min(n, 255u);
 }
 
+template <bool MIXED, bool F32>
 __global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restrict__ Ps, GradConsts G,
                                                      const float* __restrict__ dl_dhdr, double* __restrict__ grad_px,
                                                      double* __restrict__ scratch, uint32_t scratch_stride,
@@ -216,7 +244,7 @@ __global__ __launch_bounds__(256) void grad_backward(const RenderParams* __restr
             const float4 wv = *reinterpret_cast<const float4*>(dl_dhdr + 4 * pix);
             const double w[3] = {(double)wv.x, (double)wv.y, (double)wv.z};
             double out[4];
-            ray_adjoint(P, G, px, py, w, sc, out);
+            ray_adjoint<MIXED, F32>(P, G, px, py, w, sc, out);
             double* o = grad_px + 4 * pix;
             o[0] = out[0];
             o[1] = out[1];
@@ -282,14 +310,27 @@ hipError_t launch_sort_desc(const uint8_t* d_keys, int n, uint32_t* d_hist, uint
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
                                 const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
-                                uint32_t n_pix, double* part, double* grad_out, hipStream_t stream) {
+                                uint32_t n_pix, double* part, double* grad_out, bool mixed_fwd, bool fp32,
+                                hipStream_t stream) {
     const int tiles_x = (W + 7) / 8, n_tiles = tiles_x * ((H + 7) / 8);
     hipLaunchKernelGGL(grad_keys, dim3((n_tiles + 3) / 4), dim3(256), 0, stream, nsteps, W, H, tiles_x, n_tiles, keys);
     hipError_t e = launch_sort_desc(keys, n_tiles, hist, order, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, stream, queue);
-    hipLaunchKernelGGL(grad_backward, dim3(threads / 256), dim3(256), 0, stream, d_params, G, dl_dhdr, grad_px,
-                       scratch, threads, queue, (uint64_t)n_tiles * 64, order);
+    const dim3 grid(threads / 256), block(256);
+    const uint64_t items = (uint64_t)n_tiles * 64;
+    if (mixed_fwd && fp32)
+        hipLaunchKernelGGL((grad_backward<true, true>), grid, block, 0, stream, d_params, G, dl_dhdr, grad_px,
+                           scratch, threads, queue, items, order);
+    else if (mixed_fwd)
+        hipLaunchKernelGGL((grad_backward<true, false>), grid, block, 0, stream, d_params, G, dl_dhdr, grad_px,
+                           scratch, threads, queue, items, order);
+    else if (fp32)
+        hipLaunchKernelGGL((grad_backward<false, true>), grid, block, 0, stream, d_params, G, dl_dhdr, grad_px,
+                           scratch, threads, queue, items, order);
+    else
+        hipLaunchKernelGGL((grad_backward<false, false>), grid, block, 0, stream, d_params, G, dl_dhdr, grad_px,
+                           scratch, threads, queue, items, order);
     hipLaunchKernelGGL(grad_reduce_part, dim3(kReduceBlocks), dim3(256), 0, stream, grad_px, n_pix, part);
     hipLaunchKernelGGL(grad_reduce_final, dim3(1), dim3(256), 0, stream, part, grad_out);
     return hipGetLastError();

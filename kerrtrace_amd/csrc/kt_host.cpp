@@ -48,7 +48,10 @@ hipError_t launch_reset_log(uint32_t* pool_head, uint32_t* overflow, uint32_t fi
 hipError_t launch_grad_backward(const RenderParams* d_params, const GradConsts& G, const float* dl_dhdr,
                                 double* grad_px, double* scratch, uint32_t threads, unsigned long long* queue,
                                 const uint32_t* nsteps, uint8_t* keys, uint32_t* hist, uint32_t* order, int W, int H,
-                                uint32_t n_pix, double* part, double* grad_out, hipStream_t stream);
+                                uint32_t n_pix, double* part, double* grad_out, bool mixed_fwd, bool fp32,
+                                hipStream_t stream);
+hipError_t launch_trace_mixed_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
+                                     DevCounters* d_counters, int blocks, hipStream_t stream);
 size_t cost_order_hist_words(int n_tiles);
 size_t bloom_h_lds_bytes(int R);
 size_t noalloc_max_pixels();
@@ -1243,12 +1246,14 @@ kt_status kt_grad_create(const kt_config* cfg, const kt_grad_opts* opts, kt_grad
     if (!cfg || !opts || !out) return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (cfg->spp != 1) return set_err(nullptr, KT_ERR_INVALID_CONFIG, "the differentiable renderer needs spp = 1");
-    if (cfg->precision != KT_PRECISION_FP64 || cfg->rhs_mode != KT_RHS_HAMILTONIAN)
+    if (cfg->rhs_mode != KT_RHS_HAMILTONIAN)
         return set_err(nullptr, KT_ERR_INVALID_CONFIG,
-                       "the differentiable renderer needs FP64 precision and the exact (Hamiltonian) right-hand side");
+                       "the differentiable renderer needs the exact (Hamiltonian) right-hand side");
     if (!(opts->fov_deg > 0.0 && opts->fov_deg < 180.0) || opts->checkpoint_interval < 1 ||
-        opts->checkpoint_interval > 256 || opts->log_steps_per_ray < 1)
-        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT, "fov in (0, 180), checkpoint interval 1..256, log size >= 1");
+        opts->checkpoint_interval > 256 || opts->log_steps_per_ray < 1 ||
+        (opts->adjoint_precision != KT_ADJOINT_FP64 && opts->adjoint_precision != KT_ADJOINT_FP32))
+        return set_err(nullptr, KT_ERR_INVALID_ARGUMENT,
+                       "fov in (0, 180), checkpoint interval 1..256, log size >= 1, adjoint precision FP64 or FP32");
     kt_grad* g = new (std::nothrow) kt_grad();
     if (!g) return set_err(nullptr, KT_ERR_DEVICE, "out of host memory");
     g->cfg = *cfg;
@@ -1393,7 +1398,9 @@ kt_status kt_grad_forward(kt_grad* g, const double theta[4], float* hdr, uint8_t
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("forward set-up: ") + hipGetErrorString(e));
     const int blocks = std::max(1, (g->cus * 8 + 3) / 4);
     e = hipEventRecord(g->ev[0], s);
-    if (e == hipSuccess) e = kt::launch_trace_fp64_record(g->d_params, g->items, 8, g->d_counters, blocks, s);
+    if (e == hipSuccess)
+        e = uses_fp32_rhs(g->cfg) ? kt::launch_trace_mixed_record(g->d_params, g->items, 8, g->d_counters, blocks, s)
+                                  : kt::launch_trace_fp64_record(g->d_params, g->items, 8, g->d_counters, blocks, s);
     if (e == hipSuccess) e = hipEventRecord(g->ev[1], s);
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("recording trace: ") + hipGetErrorString(e));
     if (hit_class) {
@@ -1433,7 +1440,8 @@ kt_status kt_grad_backward(kt_grad* g, const float* dl_dhdr, double* grad, doubl
     if (e == hipSuccess)
         e = kt::launch_grad_backward(g->d_params, G, dl_dhdr, grad_px ? grad_px : g->d_grad_px, g->d_scratch,
                                      g->threads, g->d_queue, g->d_nsteps, g->d_rkeys, g->d_rhist, g->d_rorder,
-                                     (int)g->cfg.width, (int)g->cfg.height, g->n_pix, g->d_part, grad, s);
+                                     (int)g->cfg.width, (int)g->cfg.height, g->n_pix, g->d_part, grad,
+                                     uses_fp32_rhs(g->cfg), g->opts.adjoint_precision == KT_ADJOINT_FP32, s);
     if (e == hipSuccess) e = hipEventRecord(g->ev[3], s);
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("adjoint: ") + hipGetErrorString(e));
     g->last_stream = s;

@@ -14,21 +14,30 @@
 
 namespace kt {
 
-template <bool BATCH>
+// RECORD: the differentiable renderer's mixed-precision forward pass (step
+// log on).
+template <bool BATCH, bool RECORD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_mixed_persistent(
     const RenderParams* __restrict__ Ps, int n_renders, uint64_t total_items, int refill_threshold,
     DevCounters* __restrict__ counters) {
-    trace_loop<BATCH, false, true>(Ps, n_renders, total_items, refill_threshold, counters);
+    trace_loop<BATCH, RECORD, true>(Ps, n_renders, total_items, refill_threshold, counters);
 }
 
 hipError_t launch_trace_mixed(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,
                               DevCounters* d_counters, int blocks, hipStream_t stream) {
     if (n_renders == 1)
-        hipLaunchKernelGGL(trace_mixed_persistent<false>, dim3(blocks), dim3(256), 0, stream, d_params, 1,
+        hipLaunchKernelGGL((trace_mixed_persistent<false, false>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
                            total_items, refill_threshold, d_counters);
     else
-        hipLaunchKernelGGL(trace_mixed_persistent<true>, dim3(blocks), dim3(256), 0, stream, d_params, n_renders,
+        hipLaunchKernelGGL((trace_mixed_persistent<true, false>), dim3(blocks), dim3(256), 0, stream, d_params, n_renders,
                            total_items, refill_threshold, d_counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_mixed_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
+                                     DevCounters* d_counters, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((trace_mixed_persistent<false, true>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
+                       total_items, refill_threshold, d_counters);
     return hipGetLastError();
 }
 

@@ -9,7 +9,7 @@ The reference renders no gradients; this is an addition of this library.
 """
 import ctypes as C
 
-from ._abi import KtGradOpts, KtGradStats
+from ._abi import KT_ADJOINT_FP64, KtGradOpts, KtGradStats
 from ._lib import check, lib
 from .config import GRAD_FOV_DEG
 
@@ -24,10 +24,14 @@ def _torch():
 class GradRenderer:
     """One differentiable renderer handle: forward(theta) then backward(dL/dhdr)."""
 
-    def __init__(self, cfg, fov_deg=GRAD_FOV_DEG, checkpoint_interval=32, log_steps_per_ray=512):
+    def __init__(self, cfg, fov_deg=GRAD_FOV_DEG, checkpoint_interval=32, log_steps_per_ray=512,
+                 adjoint_precision=KT_ADJOINT_FP64):
+        """cfg.precision chooses the forward tracer (FP64 or mixed);
+        adjoint_precision the reverse pass (KT_ADJOINT_FP64 or _FP32)."""
         _torch()
         self.cfg = cfg.copy()
-        self.opts = KtGradOpts(float(fov_deg), int(checkpoint_interval), int(log_steps_per_ray))
+        self.opts = KtGradOpts(float(fov_deg), int(checkpoint_interval), int(log_steps_per_ray),
+                               int(adjoint_precision), 0)
         h = C.c_void_p()
         check(lib().kt_grad_create(C.byref(self.cfg), C.byref(self.opts), C.byref(h)))
         self._h = h
@@ -187,7 +191,8 @@ def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.0
     return list(best[1]), history
 
 
-def autotune_checkpoint(cfg, theta, intervals=(8, 16, 32, 64), fov_deg=GRAD_FOV_DEG, log_steps_per_ray=512, reps=2):
+def autotune_checkpoint(cfg, theta, intervals=(8, 16, 32, 64), fov_deg=GRAD_FOV_DEG, log_steps_per_ray=512, reps=2,
+                        adjoint_precision=KT_ADJOINT_FP64):
     """Checkpoint interval whose adjoint pass is fastest for this scene:
     for each interval a renderer traces theta and runs the backward pass
     (unit loss weights) `reps` times.  Returns (best interval, {interval:
@@ -195,7 +200,8 @@ def autotune_checkpoint(cfg, theta, intervals=(8, 16, 32, 64), fov_deg=GRAD_FOV_
     torch = _torch()
     times = {}
     for k in intervals:
-        with GradRenderer(cfg, fov_deg=fov_deg, checkpoint_interval=k, log_steps_per_ray=log_steps_per_ray) as g:
+        with GradRenderer(cfg, fov_deg=fov_deg, checkpoint_interval=k, log_steps_per_ray=log_steps_per_ray,
+                          adjoint_precision=adjoint_precision) as g:
             hdr, _ = g.forward(theta)
             w = torch.ones_like(hdr)
             best = float("inf")

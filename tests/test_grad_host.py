@@ -22,7 +22,6 @@ def _ok_opts():
 
 @pytest.mark.parametrize("field,value,needle", [
     ("spp", 2, "spp"),
-    ("precision", K.KT_PRECISION_MIXED, "FP64"),
     ("rhs_mode", K.KT_RHS_REFERENCE, "Hamiltonian"),
 ])
 def test_grad_create_rejects_uncovered_configs(field, value, needle):
@@ -34,7 +33,7 @@ def test_grad_create_rejects_uncovered_configs(field, value, needle):
 
 
 @pytest.mark.parametrize("opts", [KtGradOpts(0.0, 32, 512), KtGradOpts(30.0, 0, 512), KtGradOpts(30.0, 300, 512),
-                                  KtGradOpts(30.0, 32, 0)])
+                                  KtGradOpts(30.0, 32, 0), KtGradOpts(30.0, 32, 512, 2), KtGradOpts(30.0, 32, 512, -1)])
 def test_grad_create_rejects_bad_options(opts):
     cfg, _ = K.preset("grad512")
     rc, h = _create(cfg, opts)
