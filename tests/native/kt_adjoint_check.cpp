@@ -27,6 +27,32 @@ void kta_rhs_vjp(const double Y[6], double pt, double a, double M, const double 
     *lam_a = la;
 }
 
+// FP32 variants of the right-hand-side VJP and of one step's reversal.
+void kta_rhs_vjp32(const double Y[6], double pt, double a, double M, const double w[6], double lamY[6],
+                   double* lam_pt, double* lam_a) {
+    V6f l = v6f_zero();
+    float lp = 0.f, la = 0.f;
+    rhs_vjp32(to_f(v6(Y)), (float)pt, (float)a, (float)(a * a), (float)M, to_f(v6(w)), l, lp, la);
+    put(to_d(l), lamY);
+    *lam_pt = lp;
+    *lam_a = la;
+}
+
+void kta_step_reverse32(const double s[6], double h, double pt, double a, double M, const double lam_ns[6],
+                        double lam_s[6], double* lam_pt, double* lam_a) {
+    Geo g{a, a * a, M, true};
+    V6 Y[5];
+    StepOut so = dp_step(v6(s), h, pt, g, Y);
+    V6f Yf[5];
+    for (int i = 0; i < 5; ++i) Yf[i] = to_f(Y[i]);
+    float lp = 0.f, la = 0.f;
+    V6f ls = dp_step_reverse32(to_f(v6(s)), to_f(so.ns), Yf, (float)h, (float)pt, (float)a, (float)(a * a), (float)M,
+                               to_f(v6(lam_ns)), nullptr, v6f_zero(), lp, la);
+    put(to_d(ls), lam_s);
+    *lam_pt = lp;
+    *lam_a = la;
+}
+
 void kta_step(const double s[6], double h, double pt, double a, double M, double ns[6]) {
     Geo g{a, a * a, M, true};
     put(dp_step(v6(s), h, pt, g, nullptr).ns, ns);

@@ -36,6 +36,8 @@ def kta():
     d, i = C.c_double, C.c_int
     L.kta_rhs_vjp.argtypes = [_pd, d, d, d, _pd, _pd, _pd, _pd]
     L.kta_step.argtypes = [_pd, d, d, d, d, _pd]
+    L.kta_rhs_vjp32.argtypes = [_pd, d, d, d, _pd, _pd, _pd, _pd]
+    L.kta_step_reverse32.argtypes = [_pd, d, d, d, d, _pd, _pd, _pd, _pd]
     L.kta_step_reverse.argtypes = [_pd, d, d, d, d, _pd, _pd, _pd, _pd]
     L.kta_crossing.argtypes = [_pd, _pd, _pd, d, i, _pd, _pd, _pd, _pd, _pd]
     L.kta_start.argtypes = [i, i, i, i, d, d, d, d, d, i, _pd, _pd]
@@ -103,6 +105,36 @@ def test_step_matches_oracle_and_reverse_matches_jvp(kta, spin):
             lhs = float(lam_ns @ tout)
             terms = np.concatenate([lam_s * ds, [lpt.value * dpt, la.value * da]])
             assert float(terms.sum()) == pytest.approx(lhs, rel=1e-10, abs=1e-12 * np.abs(terms).sum())
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.7, 0.998])
+def test_fp32_vjp_and_step_reverse_match_oracle_jvp(kta, spin):
+    """The FP32 reversal's building blocks against the oracle's FP64 JVP:
+    <w, J v> = <J^T w, v> to FP32 accuracy (relative 1e-5 of the terms'
+    magnitudes: a few ulp of float over a dot product of 8 terms).  A
+    wrong sign or index in the FP32 dual pass fails at O(1)."""
+    rng = np.random.default_rng(int(31 + 100 * spin))
+    cfg = _cfg(spin)
+    for _ in range(10):
+        s, pt = _state(rng, spin)
+        w = rng.normal(size=6)
+        lam = np.zeros(6)
+        lpt, la = C.c_double(), C.c_double()
+        kta.kta_rhs_vjp32(_p(s), pt, spin, 1.0, _p(w), _p(lam), C.byref(lpt), C.byref(la))
+        h = float(rng.uniform(0.05, 1.0))
+        lam_ns = rng.normal(size=6)
+        lam_s = np.zeros(6)
+        spt, sa = C.c_double(), C.c_double()
+        kta.kta_step_reverse32(_p(s), h, pt, spin, 1.0, _p(lam_ns), _p(lam_s), C.byref(spt), C.byref(sa))
+        for _ in range(3):
+            ds, dpt, da = rng.normal(size=6), rng.normal(), rng.normal()
+            _, tout = O.rhs_jvp(cfg, s, pt, ds, dpt, da)
+            terms = np.concatenate([lam * ds, [lpt.value * dpt, la.value * da]])
+            assert float(terms.sum()) == pytest.approx(float(w @ tout), rel=1e-5, abs=1e-5 * np.abs(terms).sum())
+            _, tstep = O.step_jvp(cfg, s, pt, h, ds, dpt, da)
+            terms = np.concatenate([lam_s * ds, [spt.value * dpt, sa.value * da]])
+            assert float(terms.sum()) == pytest.approx(float(lam_ns @ tstep), rel=1e-5,
+                                                       abs=1e-5 * np.abs(terms).sum())
 
 
 @pytest.mark.parametrize("root_mode", [K.KT_DISK_ROOT_TRUE, K.KT_DISK_ROOT_SECANT_REF])
