@@ -233,12 +233,12 @@ KT_HDI StepOut dp_step(const V6& s, double hs, double pt, const Geo& g, V6* Y) {
     const V6 k5 = rhs_fwd<MIXED>(y5, pt, g);
     const V6 y6 = KTA_V(KTA_ST6);
 #define KTA_B4ACC(c) fma(KTA_B5, k5.c, fma(KTA_B4, k4.c, fma(KTA_B3, k3.c, KTA_B1 * k1.c)))
-#define KTA_D5ACC(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c)
+#define KTA_D5ACC(c) fma(Dense::d5, k5.c, fma(Dense::d4, k4.c, fma(Dense::d3, k3.c, Dense::d1 * k1.c)))
     const V6 bacc = KTA_V(KTA_B4ACC);
     const V6 dacc = KTA_V(KTA_D5ACC);
     const V6 k6 = rhs_fwd<MIXED>(y6, pt, g);
 #define KTA_NS(c) fma(hs, fma(KTA_B6, k6.c, bacc.c), s.c)
-#define KTA_D6ACC(c) (dacc.c + Dense::d6 * k6.c)
+#define KTA_D6ACC(c) fma(Dense::d6, k6.c, dacc.c)
     o.ns = KTA_V(KTA_NS);
     o.dacc6 = KTA_V(KTA_D6ACC);
     o.k1 = k1;

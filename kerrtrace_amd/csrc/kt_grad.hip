@@ -72,8 +72,7 @@ static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& 
     // the step's dense-output ingredients, as the forward pass formed them
     const V6 k1 = rhs_fwd<MIXED>(sn, pt, g), k3 = rhs_fwd<MIXED>(y3, pt, g), k4 = rhs_fwd<MIXED>(y4, pt, g);
     const V6 k5 = rhs_fwd<MIXED>(y5, pt, g), k6 = rhs_fwd<MIXED>(y6, pt, g), k7 = rhs_fwd<MIXED>(ns, pt, g);
-#define KT_DS(c) ((Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c) + Dense::d6 * k6.c) + \
-                     Dense::d7 * k7.c
+#define KT_DS(c) dense_sum(k1.c, k3.c, k4.c, k5.c, k6.c, k7.c)
     const V6 dsum{KT_DS(x), KT_DS(y), KT_DS(z), KT_DS(px), KT_DS(py), KT_DS(pz)};
 #undef KT_DS
     const Crossing hit = crossing_hit(sn, ns, k1, k7, dsum, hs, P.disk_root_mode);

@@ -132,8 +132,9 @@ struct Geo {
 // grad(H) = dr/dx_j HR + (0, HY, 0) (the reference's as-written dr/dy,
 // whose u/s term is halved, enters through w2).  Products that feed a sum
 // are fused with explicit fma; the reciprocals 1/s, 1/r, 1/(r^2+a^2),
-// 1/(r^4+a^2y^2) come from one division.
-KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
+// 1/(r^4+a^2y^2) come from one division.  r_out (optional) receives the
+// Kerr-Schild radius of st, the same value ks_r() returns.
+KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
     const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
     const double a2 = g.aa, a = g.a;
     double rho2 = fma(x, x, fma(y, y, z * z));
@@ -143,6 +144,7 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g) {
     double s = sqrt_pos(s_sq);
     double r2 = 0.5 * (u + s);
     double r = sqrt_pos(r2);
+    if (r_out) *r_out = r;
     double denom = r2 + a2;
     double denomH = fma(r2, r2, ya * y);
     double sr = s * r;
@@ -307,14 +309,18 @@ struct Dense {
     static constexpr double d6 = -1453857185.0 / 822651844.0;
     static constexpr double d7 = 69997945.0 / 29380423.0;
 };
+// d1 k1 + d3 k3 + d4 k4 + d5 k5 + d6 k6 + d7 k7 as one fma chain in stage
+// order (the tracer folds it the same way while its stages run).
+KT_HDI double dense_sum(double k1, double k3, double k4, double k5, double k6, double k7) {
+    return fma(Dense::d7, k7, fma(Dense::d6, k6, fma(Dense::d5, k5, fma(Dense::d4, k4, fma(Dense::d3, k3, Dense::d1 * k1)))));
+}
 KT_HDI double dense(double y0, double y1, double k1, double k3, double k4, double k5,
                                         double k6, double k7, double h, double t) {
     double r1 = y0;
     double r2 = y1 - y0;
     double r3 = y0 + h * k1 - y1;
     double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
-    double r5 = h * (Dense::d1 * k1 + Dense::d3 * k3 + Dense::d4 * k4 + Dense::d5 * k5 + Dense::d6 * k6 +
-                     Dense::d7 * k7);
+    double r5 = h * dense_sum(k1, k3, k4, k5, k6, k7);
     double t2 = t * t;
     double t3 = t2 * t;
     double t4 = t2 * t2;
@@ -325,17 +331,15 @@ KT_HDI double dense_dt(double y0, double y1, double k1, double k3, double k4, do
     double r2 = y1 - y0;
     double r3 = y0 + h * k1 - y1;
     double r4 = 2.0 * (y1 - y0) - h * (k1 + k7);
-    double r5 = h * (Dense::d1 * k1 + Dense::d3 * k3 + Dense::d4 * k4 + Dense::d5 * k5 + Dense::d6 * k6 +
-                     Dense::d7 * k7);
+    double r5 = h * dense_sum(k1, k3, k4, k5, k6, k7);
     double t2 = t * t;
     double t3 = t2 * t;
     return (r2 + r3) - 2.0 * t * (r3 - r4 - r5) - 3.0 * t2 * (r4 + 2.0 * r5) + 4.0 * t3 * r5;
 }
 
 // The same interpolant from its three step-dependent ingredients: k1, k7
-// and dsum = d1 k1 + d3 k3 + d4 k4 + d5 k5 + d6 k6 + d7 k7 (summed left to
-// right, as dense() does), so a tracer can fold the sum while it runs the
-// stages instead of keeping every stage derivative.
+// and dsum = dense_sum(k1, k3, ..., k7), so a tracer can fold the sum while
+// it runs the stages instead of keeping every stage derivative.
 KT_HDI double dense_s(double y0, double y1, double k1, double k7, double dsum, double h,
                                           double t) {
     double r1 = y0;

@@ -123,15 +123,15 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
 #define KT_E5ACC(c) fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))
             V6 bacc = KT_STAGE(KT_B4ACC);
             eacc = KT_STAGE(KT_E5ACC);
-            // dense-output sum d1 k1 + d3 k3 + ... + d7 k7, left to right
-#define KT_D5ACC(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c)
+            // dense-output sum d1 k1 + d3 k3 + ... + d7 k7 as an fma chain (dense_sum)
+#define KT_D5ACC(c) fma(Dense::d5, k5.c, fma(Dense::d4, k4.c, fma(Dense::d3, k3.c, Dense::d1 * k1.c)))
             dacc = KT_STAGE(KT_D5ACC);
             const V6 k6 = KT_RHS(in6);
 #define KT_NS(c) fma(hs, fma(KT_B6, k6.c, bacc.c), s.c)
 #define KT_EACC6(c) fma(KT_E6, k6.c, eacc.c)
             ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
             eacc = KT_STAGE(KT_EACC6);
-#define KT_D6ACC(c) (dacc.c + Dense::d6 * k6.c)
+#define KT_D6ACC(c) fma(Dense::d6, k6.c, dacc.c)
             dacc = KT_STAGE(KT_D6ACC);
 #undef KT_ST2
 #undef KT_ST3
@@ -145,10 +145,14 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
 #undef KT_NS
 #undef KT_EACC6
         }
-        const V6 k7 = KT_RHS(ns);
+        // k7 = f(ns); the FP64 right-hand side also yields the Kerr-Schild
+        // radius of ns, which the commit's horizon and escape tests need
+        double r_ns = 0.0;
+        const V6 k7 = MIXED ? rhs32(ns, ptf, g, g32) : rhs(ns, pt, g, &r_ns);
+        if (MIXED) r_ns = ks_r(ns.x, ns.y, ns.z, g.aa);
 #undef KT_RHS
         cnt.rhs += 6;
-#define KT_D7ACC(c) (dacc.c + Dense::d7 * k7.c)
+#define KT_D7ACC(c) fma(Dense::d7, k7.c, dacc.c)
         dacc = KT_STAGE(KT_D7ACC);
 #undef KT_D7ACC
         // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
@@ -179,7 +183,7 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
         if (accepted) L.k1 = k7;  // FSAL
         L.have_k1 = accepted;
         if (RECORD) record_step(L, prm<BATCH>(Ps, L.rid), s, hs, lane);
-        commit(L, s, ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
+        commit(L, s, ns, r_ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
     }
 
     flush_counters(cnt, counters, lane);

@@ -516,13 +516,14 @@ __device__ __forceinline__ void post_commit(Lane& L, const RenderParams& P, cons
 // test, then -- on a change of sign of y -- the crossing parameter on the
 // step's DOPRI5 dense output (the clamped secant as written, or the FP64
 // Newton/bisection root) and the interpolated hit state.  k1, k7 and d
-// (sum_i d_i k_i) are the step's dense-output ingredients.  The emission
+// (sum_i d_i k_i) are the step's dense-output ingredients; r_ns is the
+// Kerr-Schild radius of ns.  The emission
 // and the rest of the step are left to post_commit.
-__device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, double hs, const V6& k1, const V6& k7,
-                                       const V6& d, const Hot& H, int disk_root_mode) {
+__device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, double r_ns, double hs, const V6& k1,
+                                       const V6& k7, const V6& d, const Hot& H, int disk_root_mode) {
     L.s = ns;
     L.att_in_step = 0;
-    L.r_now = ks_r(ns.x, ns.y, ns.z, H.g.aa);
+    L.r_now = r_ns;  // ks_r(ns), from the right-hand side at ns
     if (L.r_now < H.rh_eps) {  // horizon, tested before the disk (kerr_trace.cuh:135-137)
         L.cls = (L.cls & ~3) | 1;
         L.finished = true;

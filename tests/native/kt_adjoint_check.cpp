@@ -76,8 +76,7 @@ void kta_step_reverse(const double s[6], double h, double pt, double a, double M
 void kta_crossing(const double s[6], const double ns[6], const double k[42], double h, int root_mode,
                   const double lam_hit[5], double hit_out[6], double lam_s[6], double lam_ns[6], double lk[36]) {
     const V6 k1 = v6(k), k3 = v6(k + 12), k4 = v6(k + 18), k5 = v6(k + 24), k6 = v6(k + 30), k7 = v6(k + 36);
-#define KTA_DS(c) (Dense::d1 * k1.c + Dense::d3 * k3.c + Dense::d4 * k4.c + Dense::d5 * k5.c + Dense::d6 * k6.c + \
-                   Dense::d7 * k7.c)
+#define KTA_DS(c) dense_sum(k1.c, k3.c, k4.c, k5.c, k6.c, k7.c)
     const V6 d{KTA_DS(x), KTA_DS(y), KTA_DS(z), KTA_DS(px), KTA_DS(py), KTA_DS(pz)};
 #undef KTA_DS
     Crossing c = crossing_hit(v6(s), v6(ns), k1, k7, d, h, root_mode);
