@@ -422,10 +422,11 @@ def copy_calibration(mib, reps=3):
     print(f"[bench] copied {mib} MiB x {reps} ({mib * 1024} KiB read and written per copy)", flush=True)
 
 
-def trace_only(spec, steps, launch_opts):
+def trace_only(spec, steps, launch_opts, lone=False):
     """Trace one workload `steps` times and print its work counters: the
     command profiles run under rocprofv3.  spec = preset[:tolerance[:fp64|mixed[:spin]]],
-    e.g. hd1080_a0998:1e-7:mixed or uhd2160_i60:1e-7:fp64:0.998."""
+    e.g. hd1080_a0998:1e-7:mixed or uhd2160_i60:1e-7:fp64:0.998.  lone:
+    trace only the frame's longest ray (lone_ray_scene)."""
     import torch
 
     from kerrtrace_amd import config as K
@@ -437,6 +438,9 @@ def trace_only(spec, steps, launch_opts):
         cfg.tolerance = float(parts[1])
     if len(parts) > 2:
         cfg.precision = {"fp64": K.KT_PRECISION_FP64, "mixed": K.KT_PRECISION_MIXED}[parts[2]]
+    if lone:
+        cfg, cam = lone_ray_scene(cfg, cam, launch_opts)
+        launch_opts = (1, 1, 1, 2, 256, 0)
     with HipRenderer(cfg) as r:
         if launch_opts:
             r.set_launch_opts(*launch_opts)
@@ -448,6 +452,46 @@ def trace_only(spec, steps, launch_opts):
                   f"{st['rhs_evals']} rhs, {st['accepted']} accepted, max {st['max_attempts_ray']} per ray",
                   flush=True)
         torch.cuda.synchronize()
+
+
+def lone_ray_scene(cfg, cam, launch_opts):
+    """A 1x1 render of the frame's longest ray: the frame is traced once
+    with per-pixel attempt counts, then the camera is aimed along the exact
+    jittered direction of its longest pixel with fov_scale 0, so the single
+    ray repeats that pixel's arithmetic (and attempts) bit for bit, alone
+    on the GPU: the frame's critical path."""
+    import torch
+
+    from kerrtrace_amd._abi import KtCamera
+    from kerrtrace_amd.renderer import HipRenderer
+    with HipRenderer(cfg) as r:
+        if launch_opts:
+            r.set_launch_opts(*launch_opts)
+        att = torch.zeros((cfg.height, cfg.width), dtype=torch.int32, device="cuda")
+        r.trace(cam, with_class=False, attempts=att)
+        a = att.cpu()
+    idx = int(a.argmax())
+    y, x = divmod(idx, cfg.width)
+
+    def pcg32(state):
+        m64 = (1 << 64) - 1
+        v = (state * 6364136223846793005 + 1442695040888963407) & m64
+        xs = (((v >> 18) ^ v) >> 27) & 0xFFFFFFFF
+        rot = v >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+    base = (y * cfg.width + x) << 32
+    rx, ry = pcg32(base) * 2.0 ** -32, pcg32(base ^ 0xda3e39cb94b95bdb) * 2.0 ** -32
+    u = ((x + rx) * (2.0 / cfg.width) - 1.0) * (cfg.width / cfg.height) * cam.fov_scale
+    v = (1.0 - (y + ry) * (2.0 / cfg.height)) * cam.fov_scale
+    one = KtCamera()
+    for k in range(3):
+        one.pos[k], one.rgt[k], one.up[k] = cam.pos[k], cam.rgt[k], cam.up[k]
+        one.fwd[k] = u * cam.rgt[k] + v * cam.up[k] + cam.fwd[k]
+    one.fov_scale = 0.0
+    c1 = cfg.copy()
+    c1.width = c1.height = 1
+    print(f"[bench] lone ray: pixel ({x}, {y}), {int(a[y, x])} attempts in the frame", flush=True)
+    return c1, one
 
 
 def main():
@@ -477,10 +521,12 @@ def main():
     ap.add_argument("--trace-only", type=str, default=None,
                     help="only trace one workload --steps times with --launch-opts and print its counters "
                          "(for profiler runs): preset[:tolerance[:fp64|mixed[:spin]]]")
+    ap.add_argument("--lone-ray", action="store_true",
+                    help="with --trace-only: trace only the frame's longest ray, alone (its critical path)")
     args = ap.parse_args()
     if args.trace_only:
         opts = [int(v) for v in args.launch_opts.split(",")] if args.launch_opts else None
-        trace_only(args.trace_only, args.steps, opts)
+        trace_only(args.trace_only, args.steps, opts, lone=args.lone_ray)
         return
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)

@@ -207,7 +207,7 @@ struct StepOut {
 template <bool MIXED>
 KT_HDI V6 rhs_fwd(const V6& st, double pt, const Geo& g) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (MIXED) return rhs32(st, (float)pt, g, G32{(float)g.a, (float)g.aa, (float)g.M, g.exact});
+    if (MIXED) return rhs32(st, (float)pt, g, make_g32(g));
 #endif
     return rhs(st, pt, g);
 }

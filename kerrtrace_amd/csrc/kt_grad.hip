@@ -68,7 +68,7 @@ static __device__ __forceinline__ CrossAdj crossing_adjoint(const RenderParams& 
                                                          double lamTr) {
     CrossAdj o;
     o.lit = false;
-    const Geo g{P.a, P.aa, P.M, true};
+    const Geo g = make_geo(P.a, P.aa, P.M, true);
     // the step's dense-output ingredients, as the forward pass formed them
     const V6 k1 = rhs_fwd<MIXED>(sn, pt, g), k3 = rhs_fwd<MIXED>(y3, pt, g), k4 = rhs_fwd<MIXED>(y4, pt, g);
     const V6 k5 = rhs_fwd<MIXED>(y5, pt, g), k6 = rhs_fwd<MIXED>(y6, pt, g), k7 = rhs_fwd<MIXED>(ns, pt, g);
@@ -108,7 +108,7 @@ __device__ __forceinline__ void ray_adjoint(const RenderParams& P, const GradCon
     const uint32_t N = LG.nsteps[pix];
     if (N == 0) return;  // no step logged (or the log overflowed)
     const uint32_t K = LG.K;
-    const Geo g{P.a, P.aa, P.M, true};
+    const Geo g = make_geo(P.a, P.aa, P.M, true);
     const double pt = LG.pk[2 * pix], kt = LG.pk[2 * pix + 1];
     const uint8_t cls = P.cls ? P.cls[pix] : 0;
     // colour adjoint W = d loss / d colour

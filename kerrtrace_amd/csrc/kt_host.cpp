@@ -31,9 +31,10 @@
 namespace kt {
 hipError_t launch_reset(DevCounters* c, uint32_t* error_flag, hipStream_t stream);
 hipError_t launch_trace_fp64(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,
-                             DevCounters* d_counters, int blocks, hipStream_t stream);
+                             DevCounters* d_counters, int blocks, bool rhs_exact, hipStream_t stream);
 hipError_t launch_trace_mixed(const RenderParams* d_params, int n_renders, uint64_t total_items,
-                              int refill_threshold, DevCounters* d_counters, int blocks, hipStream_t stream);
+                              int refill_threshold, DevCounters* d_counters, int blocks, bool rhs_exact,
+                              hipStream_t stream);
 hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, const float* d_wts,
                           const float* d_ws_col, float threshold, hipStream_t stream);
 hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, int W, int H, int R, int K, int waves,
@@ -446,8 +447,8 @@ int trace_into(kt_renderer* r, const kt_camera* cam, float* hdr, uint8_t* cls, u
     int thr = r->opts.refill_threshold > 0 ? std::min(r->opts.refill_threshold, 64) : 8;
     KT_HIP(r, hipEventRecord(sl.ev0, s), "trace event");
     hipError_t e = uses_fp32_rhs(r->cfg)
-                       ? kt::launch_trace_mixed(dp, 1, items, thr, sl.counters, blocks, s)
-                       : kt::launch_trace_fp64(dp, 1, items, thr, sl.counters, blocks, s);
+                       ? kt::launch_trace_mixed(dp, 1, items, thr, sl.counters, blocks, P.rhs_exact != 0, s)
+                       : kt::launch_trace_fp64(dp, 1, items, thr, sl.counters, blocks, P.rhs_exact != 0, s);
     if (e != hipSuccess) return set_err(r, KT_ERR_LAUNCH, std::string("trace launch: ") + hipGetErrorString(e));
     KT_HIP(r, hipEventRecord(sl.done, s), "trace event");
     // the ring slot's event now follows the kernels that read the record, so
@@ -1198,8 +1199,10 @@ kt_status kt_batch_trace(kt_batch* b, float* const* hdr, uint8_t* const* cls, vo
     if (e != hipSuccess) return hip_err(nullptr, e, "trace event");
     bool fp32_rhs = true;  // one kernel for the batch: FP64 if any render needs it
     for (int i = 0; i < b->n; ++i) fp32_rhs = fp32_rhs && uses_fp32_rhs(b->cfgs[i]);
-    e = fp32_rhs ? kt::launch_trace_mixed(b->d_params, b->n, items, thr, b->d_counters, blocks, s)
-                                           : kt::launch_trace_fp64(b->d_params, b->n, items, thr, b->d_counters, blocks, s);
+    // a batch of one is a single render: its own right-hand-side form
+    const bool exact1 = b->cfgs[0].rhs_mode == KT_RHS_HAMILTONIAN;
+    e = fp32_rhs ? kt::launch_trace_mixed(b->d_params, b->n, items, thr, b->d_counters, blocks, exact1, s)
+                 : kt::launch_trace_fp64(b->d_params, b->n, items, thr, b->d_counters, blocks, exact1, s);
     if (e != hipSuccess) return set_err(nullptr, KT_ERR_LAUNCH, std::string("batch trace launch: ") + hipGetErrorString(e));
     e = hipEventRecord(b->ev1, s);
     if (e != hipSuccess) return hip_err(nullptr, e, "trace event");

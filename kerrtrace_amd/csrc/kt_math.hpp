@@ -34,17 +34,17 @@ KT_HDI uint32_t pcg32(uint64_t state) {
 // x = m 2^e, m in [1,2), e = 5q + j: x^(-1/5) = m^(-1/5) 2^(-j/5) 2^(-q).
 // m^(-1/5): quadratic start (|rel. error| < 2e-3), then four Newton steps
 // y <- y (6 - m y^5) / 5 on y^-5 = m (error 2e-3 -> 1e-5 -> 4e-10 -> 5e-19).
+// Branch-free: the main path runs for every x and the special values
+// (NaN, 0, infinities, negatives) are selected at the end, so a wave takes
+// no branches here; q = floor(e / 5) comes from an unsigned division of
+// e + 1075 >= 1 (e >= -1074 for subnormals).
 KT_HDI double pow_m02(double x) {
-    if (x != x) return x;
-    if (x == 0.0) return __builtin_huge_val();
-    if (__builtin_isinf(x)) return x > 0.0 ? 0.0 : __builtin_nan("");
-    if (x < 0.0) return __builtin_nan("");
     int e = 0;
     double m = frexp(x, &e);
     m = m * 2.0;
     e = e - 1;
-    int q = e >= 0 ? e / 5 : -((-e + 4) / 5);
-    int j = e - 5 * q;
+    const int q = (int)((unsigned)(e + 1075) / 5u) - 215;
+    const int j = e - 5 * q;
     double tj = 1.0;
     tj = j == 1 ? 0x1.bdb8cdadbe120p-1 : tj;   // 2^(-1/5)
     tj = j == 2 ? 0x1.8406003b2ae5cp-1 : tj;   // 2^(-2/5)
@@ -57,7 +57,12 @@ KT_HDI double pow_m02(double x) {
         double y5 = y2 * y2 * y;
         y = y * (6.0 - m * y5) * 0.2;
     }
-    return ldexp(y * tj, -q);
+    double out = ldexp(y * tj, -q);
+    out = x < 0.0 ? __builtin_nan("") : out;
+    out = x == __builtin_huge_val() ? 0.0 : out;
+    out = x == 0.0 ? __builtin_huge_val() : out;
+    out = x != x ? x : out;
+    return out;
 }
 
 // Correctly rounded FP64 square root and quotient for the operands the
@@ -122,7 +127,14 @@ struct V6 {
 struct Geo {
     double a, aa, M;
     bool exact;  // exact Hamiltonian dr^2/dy instead of the as-written one
+    // dr/dy weight w2 = fma(u + w2c, w2f / s, 1) / (2 r): exact form
+    // (w2c, w2f) = (2 a^2, 1), the reference's as-written form (4 a^2, 1/2);
+    // both products are exact scalings, so either form rounds as written
+    double w2c, w2f;
 };
+KT_HDI Geo make_geo(double a, double aa, double M, bool exact) {
+    return Geo{a, aa, M, exact, exact ? 2.0 * aa : 4.0 * aa, exact ? 1.0 : 0.5};
+}
 
 // cuda/kerr_math.cuh:82-156: Hamiltonian right-hand side in Cartesian
 // Kerr-Schild coordinates, dx/dl = p - 2H (l.p - pt) l and
@@ -169,7 +181,7 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
     d.z = fma(-c, lz, pz);
     double inv_2r = 0.5 * inv_r;
     double w1 = fma(u, inv_s, 1.0) * inv_2r;
-    double w2 = g.exact ? fma(u + 2.0 * a2, inv_s, 1.0) * inv_2r : fma(u + 4.0 * a2, 0.5 * inv_s, 1.0) * inv_2r;
+    double w2 = fma(u + g.w2c, g.w2f * inv_s, 1.0) * inv_2r;
     double Q = fma(lx, px, lz * pz);
     double XP = fma(x, px, z * pz);
     double inv_r2 = inv_r * inv_r;
@@ -198,8 +210,12 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
 // FP64 see no O(1) FP32 rounding.
 struct G32 {
     float a, aa, M;
-    bool exact;
+    float w2c, w2f;  // Geo::w2c, Geo::w2f in FP32
 };
+KT_HDI G32 make_g32(const Geo& g) {
+    const float aa = (float)g.aa;
+    return G32{(float)g.a, aa, (float)g.M, g.exact ? 2.0f * aa : 4.0f * aa, g.exact ? 1.0f : 0.5f};
+}
 __device__ __forceinline__ V6 rhs32(const V6& st, float ptf, const Geo& g, const G32& gf) {
     const double x = st.x, y = st.y, z = st.z;
     // FP64 radius solve
@@ -232,8 +248,7 @@ __device__ __forceinline__ V6 rhs32(const V6& st, float ptf, const Geo& g, const
     const float c = (2.0f * H) * lp;
     const float inv_2r = 0.5f * inv_r;
     const float w1 = fmaf(uf, inv_s, 1.0f) * inv_2r;
-    const float w2 = gf.exact ? fmaf(uf + 2.0f * a2, inv_s, 1.0f) * inv_2r
-                              : fmaf(uf + 4.0f * a2, 0.5f * inv_s, 1.0f) * inv_2r;
+    const float w2 = fmaf(uf + gf.w2c, gf.w2f * inv_s, 1.0f) * inv_2r;
     const float XP = fmaf(xf, pxf, zf * pzf);
     const float inv_r2 = inv_r * inv_r;
     const float G = fmaf(fmaf(-2.0f * r, Q, XP), inv_denom, -((yf * pyf) * inv_r2));

@@ -10,11 +10,11 @@
 
 namespace kt {
 
-template <bool BATCH, bool RECORD>
+template <bool BATCH, bool RECORD, int RHS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_fp64_persistent(
     const RenderParams* __restrict__ Ps, int n_renders, uint64_t total_items, int refill_threshold,
     DevCounters* __restrict__ counters) {
-    trace_loop<BATCH, RECORD, false>(Ps, n_renders, total_items, refill_threshold, counters);
+    trace_loop<BATCH, RECORD, false, RHS>(Ps, n_renders, total_items, refill_threshold, counters);
 }
 
 __global__ void reset_counters(DevCounters* c, uint32_t* error_flag) {
@@ -41,23 +41,29 @@ hipError_t launch_reset(DevCounters* c, uint32_t* error_flag, hipStream_t stream
     return hipGetLastError();
 }
 
-// d_params holds n_renders records sorted by ray_begin (one for a single render).
+// d_params holds n_renders records sorted by ray_begin (one for a single
+// render).  rhs_exact: the single render's right-hand-side form (a batch
+// reads each render's own).
 hipError_t launch_trace_fp64(const RenderParams* d_params, int n_renders, uint64_t total_items, int refill_threshold,
-                             DevCounters* d_counters, int blocks, hipStream_t stream) {
-    if (n_renders == 1)
-        hipLaunchKernelGGL((trace_fp64_persistent<false, false>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
-                           total_items, refill_threshold, d_counters);
+                             DevCounters* d_counters, int blocks, bool rhs_exact, hipStream_t stream) {
+    if (n_renders > 1)
+        hipLaunchKernelGGL((trace_fp64_persistent<true, false, kRhsPerRender>), dim3(blocks), dim3(256), 0, stream,
+                           d_params, n_renders, total_items, refill_threshold, d_counters);
+    else if (rhs_exact)
+        hipLaunchKernelGGL((trace_fp64_persistent<false, false, kRhsExact>), dim3(blocks), dim3(256), 0, stream,
+                           d_params, 1, total_items, refill_threshold, d_counters);
     else
-        hipLaunchKernelGGL((trace_fp64_persistent<true, false>), dim3(blocks), dim3(256), 0, stream, d_params,
-                           n_renders, total_items, refill_threshold, d_counters);
+        hipLaunchKernelGGL((trace_fp64_persistent<false, false, kRhsAsWritten>), dim3(blocks), dim3(256), 0, stream,
+                           d_params, 1, total_items, refill_threshold, d_counters);
     return hipGetLastError();
 }
 
-// The differentiable renderer's forward pass (one render, step log on).
+// The differentiable renderer's forward pass (one render, step log on; the
+// differentiable renderer takes only the exact right-hand side).
 hipError_t launch_trace_fp64_record(const RenderParams* d_params, uint64_t total_items, int refill_threshold,
                                     DevCounters* d_counters, int blocks, hipStream_t stream) {
-    hipLaunchKernelGGL((trace_fp64_persistent<false, true>), dim3(blocks), dim3(256), 0, stream, d_params, 1,
-                       total_items, refill_threshold, d_counters);
+    hipLaunchKernelGGL((trace_fp64_persistent<false, true, kRhsExact>), dim3(blocks), dim3(256), 0, stream, d_params,
+                       1, total_items, refill_threshold, d_counters);
     return hipGetLastError();
 }
 

@@ -37,7 +37,12 @@ namespace kt {
 // MIXED: the mixed-precision tracer: the right-hand sides run in rhs32
 // (FP64 radius solve, FP32 bulk); the stage algebra, state, error norm,
 // controller, dense output and shading stay FP64.
-template <bool BATCH, bool RECORD, bool MIXED>
+// RHS: how the right-hand side's dr/dy weight (Geo::w2c, w2f) is chosen:
+// from each lane's render (batched launches), or fixed for the launch, so
+// the exact form's unit factor and the as-written form's 1/2 fold away.
+enum : int { kRhsPerRender = 0, kRhsExact = 1, kRhsAsWritten = 2 };
+
+template <bool BATCH, bool RECORD, bool MIXED, int RHS>
 __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, int n_renders, uint64_t total_items,
                                            int refill_threshold, DevCounters* __restrict__ counters) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -55,9 +60,11 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
     LaneCounters cnt;
     bool drained = false;
     int prio = 0;
+    uint32_t turn = 0;  // loop turns (wave-uniform)
     const uint32_t long_ray = prm<BATCH>(Ps, 0).long_ray;
     const int isolate = (int)prm<BATCH>(Ps, 0).isolate;
     Hot H = load_hot(prm<BATCH>(Ps, 0));
+    if (!BATCH) hot_to_vgprs(H);  // (a batch's lanes load their own)
     if (lane == 0) atomicMin(&counters->clock_start, (unsigned long long)wall_clock64());
 
     for (;;) {
@@ -76,63 +83,68 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
         }
         // ---- refill idle lanes from the work queue (wave-uniform branch)
         // (a wave isolated around a long ray takes no new rays until it ends)
-        if (isolate == 0 || prio < isolate)
+        if (!uniform_flag(drained) && (isolate == 0 || prio < isolate))
             refill<BATCH>(L, H, Ps, n_renders, total_items, refill_threshold, counters, lane, drained);
 
-        if (drained && __ballot(L.active) == 0) break;
+        if (uniform_flag(drained) && __ballot(L.active) == 0) break;
         // a wave holding a long-running ray (photon-ring orbits) issues
         // ahead of the other wave(s) on its SIMD, so its serial chain of
-        // steps, which bounds the frame time, is not slowed by them
-        wave_priority(L, long_ray, prio);
-        if (!L.active || L.finished) continue;
-
-        // ---- one DP5(4) attempt (cuda/kerr_trace.cuh:83-122)
-        const Geo g = H.g;
-        const V6 s = L.s;
-        const G32 g32{(float)g.a, (float)g.aa, (float)g.M, g.exact};
-        const float ptf = (float)L.pt;
-        const double pt = L.pt;
+        // steps, which bounds the frame time, is not slowed by them; the
+        // thresholds are hundreds of attempts apart, so every 8th turn is
+        // often enough (the check costs three ballots)
+        if ((++turn & 7u) == 0u) wave_priority(L, long_ray, prio);
+        // (no divergent `continue` or `break` in this loop: lanes without
+        // an attempt this turn skip the block, so the loop's only exit is
+        // the wave-uniform one above and no exec masks are carried round)
+        if (L.active && !L.finished) {
+            // ---- one DP5(4) attempt (cuda/kerr_trace.cuh:83-122)
+            Geo g = H.g;
+            if (RHS != kRhsPerRender) g = make_geo(g.a, g.aa, g.M, RHS == kRhsExact);
+            const V6 s = L.s;
+            const G32 g32 = make_g32(g);
+            const float ptf = (float)L.pt;
+            const double pt = L.pt;
 #define KT_RHS(st) (MIXED ? rhs32((st), ptf, g, g32) : rhs((st), pt, g))
-        if (!L.have_k1) {
-            L.k1 = KT_RHS(s);
-            L.have_k1 = true;
-            cnt.rhs++;
-        }
-        const double hs = L.h;
-        V6 ns, eacc, dacc;
-        const V6 k1s = L.k1;
-        {
-            const V6 k1 = L.k1;
-            // stage inputs s + h * sum_j a_ij k_j as fma chains
+            if (!L.have_k1) {
+                L.k1 = KT_RHS(s);
+                L.have_k1 = true;
+                cnt.rhs++;
+            }
+            const double hs = L.h;
+            V6 ns, eacc, dacc;
+            const V6 k1s = L.k1;
+            {
+                const V6 k1 = L.k1;
+                // stage inputs s + h * sum_j a_ij k_j as fma chains
 #define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
 #define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
 #define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
 #define KT_ST5(c) fma(hs, fma(KT_A54, k4.c, fma(KT_A53, k3.c, fma(KT_A52, k2.c, KT_A51 * k1.c))), s.c)
 #define KT_ST6(c) \
-    fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
+        fma(hs, fma(KT_A65, k5.c, fma(KT_A64, k4.c, fma(KT_A63, k3.c, fma(KT_A62, k2.c, KT_A61 * k1.c)))), s.c)
 #define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
-            const V6 k2 = KT_RHS(KT_STAGE(KT_ST2));
-            const V6 k3 = KT_RHS(KT_STAGE(KT_ST3));
-            const V6 k4 = KT_RHS(KT_STAGE(KT_ST4));
-            const V6 k5 = KT_RHS(KT_STAGE(KT_ST5));
-            const V6 in6 = KT_STAGE(KT_ST6);
-            // The 5th-order solution b.k and the error (b - bhat).k are fma
-            // chains in stage order; their leading terms are folded now so
-            // k2..k5 are dead while stage 6 and 7 are evaluated.
+                const V6 k2 = KT_RHS(KT_STAGE(KT_ST2));
+                const V6 k3 = KT_RHS(KT_STAGE(KT_ST3));
+                const V6 k4 = KT_RHS(KT_STAGE(KT_ST4));
+                const V6 k5 = KT_RHS(KT_STAGE(KT_ST5));
+                const V6 in6 = KT_STAGE(KT_ST6);
+                // The 5th-order solution b.k and the error (b - bhat).k are fma
+                // chains in stage order; their leading terms are folded now so
+                // k2..k5 are dead while stage 6 and 7 are evaluated.
 #define KT_B4ACC(c) fma(KT_B5, k5.c, fma(KT_B4, k4.c, fma(KT_B3, k3.c, KT_B1 * k1.c)))
 #define KT_E5ACC(c) fma(KT_E5, k5.c, fma(KT_E4, k4.c, fma(KT_E3, k3.c, KT_E1 * k1.c)))
-            V6 bacc = KT_STAGE(KT_B4ACC);
-            eacc = KT_STAGE(KT_E5ACC);
-            // dense-output sum d1 k1 + d3 k3 + ... + d7 k7 as an fma chain (dense_sum)
+                V6 bacc = KT_STAGE(KT_B4ACC);
+                eacc = KT_STAGE(KT_E5ACC);
+                // dense-output sum d1 k1 + d3 k3 + ... + d7 k7 as an fma chain (dense_sum)
 #define KT_D5ACC(c) fma(Dense::d5, k5.c, fma(Dense::d4, k4.c, fma(Dense::d3, k3.c, Dense::d1 * k1.c)))
-            dacc = KT_STAGE(KT_D5ACC);
-            const V6 k6 = KT_RHS(in6);
+                dacc = KT_STAGE(KT_D5ACC);
+                const V6 k6 = KT_RHS(in6);
 #define KT_NS(c) fma(hs, fma(KT_B6, k6.c, bacc.c), s.c)
 #define KT_EACC6(c) fma(KT_E6, k6.c, eacc.c)
-            ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
-            eacc = KT_STAGE(KT_EACC6);
+                ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
+                eacc = KT_STAGE(KT_EACC6);
 #define KT_D6ACC(c) fma(Dense::d6, k6.c, dacc.c)
-            dacc = KT_STAGE(KT_D6ACC);
+                dacc = KT_STAGE(KT_D6ACC);
 #undef KT_ST2
 #undef KT_ST3
 #undef KT_ST4
@@ -144,46 +156,49 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
 #undef KT_D6ACC
 #undef KT_NS
 #undef KT_EACC6
-        }
-        // k7 = f(ns); the FP64 right-hand side also yields the Kerr-Schild
-        // radius of ns, which the commit's horizon and escape tests need
-        double r_ns = 0.0;
-        const V6 k7 = MIXED ? rhs32(ns, ptf, g, g32) : rhs(ns, pt, g, &r_ns);
-        if (MIXED) r_ns = ks_r(ns.x, ns.y, ns.z, g.aa);
+            }
+            // k7 = f(ns); the FP64 right-hand side also yields the Kerr-Schild
+            // radius of ns, which the commit's horizon and escape tests need
+            double r_ns = 0.0;
+            const V6 k7 = MIXED ? rhs32(ns, ptf, g, g32) : rhs(ns, pt, g, &r_ns);
+            if (MIXED) r_ns = ks_r(ns.x, ns.y, ns.z, g.aa);
 #undef KT_RHS
-        cnt.rhs += 6;
+            cnt.rhs += 6;
 #define KT_D7ACC(c) fma(Dense::d7, k7.c, dacc.c)
-        dacc = KT_STAGE(KT_D7ACC);
+            dacc = KT_STAGE(KT_D7ACC);
 #undef KT_D7ACC
-        // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
-        const double tol = L.tol;
+            // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
+            const double tol = L.tol;
 #define KT_ERR(c) div_pos(fabs(hs * fma(KT_E7, k7.c, eacc.c)), tol * fmax(fabs(s.c), 1.0))
-        double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
-        double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
+            double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
+            double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
 #undef KT_ERR
 #undef KT_STAGE
-        double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
-        if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
-            // a non-finite step error: the library quotients (special values)
+            double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
+            if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
+                // a non-finite step error: the library quotients (special values)
 #define KT_ERRX(c) (fabs(hs * fma(KT_E7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
-            err = fmax(fmax(fmax(KT_ERRX(x), KT_ERRX(y)), fmax(KT_ERRX(z), KT_ERRX(px))),
-                       fmax(KT_ERRX(py), KT_ERRX(pz)));
+                err = fmax(fmax(fmax(KT_ERRX(x), KT_ERRX(y)), fmax(KT_ERRX(z), KT_ERRX(px))),
+                           fmax(KT_ERRX(py), KT_ERRX(pz)));
 #undef KT_ERRX
+            }
+            cnt.attempts++;
+            L.ray_att++;
+            L.att_in_step++;
+            const bool accepted = err <= 1.0;
+            // controller without clamps (kerr_trace.cuh:119)
+            L.h = hs * 0.9 * pow_m02(err);
+            // a rejected attempt is retried from the same state, unless it was
+            // the step's last allowed one
+            if (accepted || L.att_in_step >= H.max_attempts) {
+                // ---- commit; a rejected last attempt is committed too (kerr_trace.cuh:123-134)
+                if (accepted) cnt.accepted++;
+                if (accepted) L.k1 = k7;  // FSAL
+                L.have_k1 = accepted;
+                if (RECORD) record_step(L, prm<BATCH>(Ps, L.rid), s, hs, lane);
+                commit(L, s, ns, r_ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
+            }
         }
-        cnt.attempts++;
-        L.ray_att++;
-        L.att_in_step++;
-        const bool accepted = err <= 1.0;
-        // controller without clamps (kerr_trace.cuh:119)
-        L.h = hs * 0.9 * pow_m02(err);
-        if (!accepted && L.att_in_step < H.max_attempts) continue;  // retry from the same state
-
-        // ---- commit; a rejected last attempt is committed too (kerr_trace.cuh:123-134)
-        if (accepted) cnt.accepted++;
-        if (accepted) L.k1 = k7;  // FSAL
-        L.have_k1 = accepted;
-        if (RECORD) record_step(L, prm<BATCH>(Ps, L.rid), s, hs, lane);
-        commit(L, s, ns, r_ns, hs, k1s, k7, dacc, H, prm<BATCH>(Ps, L.rid).disk_root_mode);
     }
 
     flush_counters(cnt, counters, lane);

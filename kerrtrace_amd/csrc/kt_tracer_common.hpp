@@ -224,7 +224,7 @@ struct Sky {
 };
 static __device__ __noinline__ Sky sky(const RenderParams& P, double x, double y, double z, double px, double py,
                                        double pz, double pt, double Tr) {
-    Geo g{P.a, P.aa, P.M, P.rhs_exact != 0};
+    const Geo g = make_geo(P.a, P.aa, P.M, P.rhs_exact != 0);
     V6 ke = rhs(V6{x, y, z, px, py, pz}, pt, g);
     double q = ke.x * ke.x + ke.y * ke.y + ke.z * ke.z;
     double inv = 1.0 / sqrt(q);
@@ -311,9 +311,26 @@ struct Hot {
     int max_steps, max_attempts, spp;
 };
 
+// A wave-uniform double moved from an SGPR pair into a VGPR pair (an empty
+// asm whose operand must sit in VGPRs).  The tracer's scalar registers are
+// the scarce ones (stage coefficients, exec masks of nested branches); the
+// thresholds that a committed step reads once cost nothing in VGPRs, while
+// spilled SGPRs cost a v_readlane per use on every step.
+__device__ __forceinline__ double in_vgpr(double x) {
+    asm("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ void hot_to_vgprs(Hot& H) {
+    H.tol = in_vgpr(H.tol);
+    H.tol_floor = in_vgpr(H.tol_floor);
+    H.cutoff = in_vgpr(H.cutoff);
+    H.rh_eps = in_vgpr(H.rh_eps);
+    H.R_esc = in_vgpr(H.R_esc);
+}
+
 __device__ __forceinline__ Hot load_hot(const RenderParams& P) {
     Hot o;
-    o.g = Geo{P.a, P.aa, P.M, P.rhs_exact != 0};
+    o.g = make_geo(P.a, P.aa, P.M, P.rhs_exact != 0);
     o.tol = P.tol;
     o.tol_floor = P.tol_floor;
     o.cutoff = P.cutoff;
@@ -529,7 +546,8 @@ __device__ __forceinline__ void commit(Lane& L, const V6& s, const V6& ns, doubl
         L.finished = true;
         return;
     }
-    L.crossing = (s.y < 0.0 && ns.y >= 0.0) || (s.y > 0.0 && ns.y <= 0.0);
+    // (bitwise & and |: no short-circuit branches)
+    L.crossing = ((s.y < 0.0) & (ns.y >= 0.0)) | ((s.y > 0.0) & (ns.y <= 0.0));
     if (L.crossing) {
         double t;
         if (disk_root_mode == 1) {
@@ -572,6 +590,12 @@ __device__ __forceinline__ void flush_counters(const LaneCounters& cnt, DevCount
     }
 }
 
+// A flag every lane of the wave holds equally, read from the first active
+// lane: the compiler then treats branches on it as wave-uniform (scalar
+// branches, no exec masks), which it cannot prove for a flag carried round
+// a loop through blocks where lanes diverge.
+__device__ __forceinline__ bool uniform_flag(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
+
 // Refill the wave's idle lanes from the work queue: one atomic per wave
 // claims as many items as there are idle lanes once `refill_threshold` of
 // them are idle (or all are), and ballot + popcount hand them out.  Items
@@ -582,13 +606,16 @@ __device__ __forceinline__ void refill(Lane& L, Hot& H, const RenderParams* __re
                                        uint64_t total_items, int refill_threshold, DevCounters* __restrict__ counters,
                                        int lane, bool& drained) {
     uint64_t idle = __ballot(!L.active);
-    if (idle == 0 || drained) return;
+    if (idle == 0) return;
     int n_idle = __popcll(idle);
     if (n_idle < refill_threshold && idle != __ballot(1)) return;
     int leader = __ffsll((unsigned long long)idle) - 1;
     unsigned long long base = 0;
     if (lane == leader) base = atomicAdd(&counters->queue_head, (unsigned long long)n_idle);
-    base = __shfl(base, leader);
+    // the leader's claim to every lane through scalar registers, so the
+    // drain test below is a scalar branch
+    base = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), leader) << 32) |
+           (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, leader);
     if (base + (unsigned long long)n_idle >= total_items) {
         drained = true;
         if (lane == leader) atomicMin(&counters->clock_drain, (unsigned long long)wall_clock64());

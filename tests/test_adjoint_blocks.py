@@ -42,7 +42,29 @@ def kta():
     L.kta_crossing.argtypes = [_pd, _pd, _pd, d, i, _pd, _pd, _pd, _pd, _pd]
     L.kta_start.argtypes = [i, i, i, i, d, d, d, d, d, i, _pd, _pd]
     L.kta_shade.argtypes = [_pd, C.POINTER(C.c_int), C.c_uint64, _pd, _pd, _pd, _pd, d, d, i, _pd, _pd]
+    L.kta_pow_m02.argtypes = [d]
+    L.kta_pow_m02.restype = d
     return L
+
+
+def test_device_pow_m02_matches_oracle(kta):
+    """The device's branch-free controller factor (special values selected
+    after the main path, q = floor(e/5) from an unsigned division) returns
+    the oracle's bits for every class of input: NaN, +-0, +-inf, negatives,
+    subnormals, the extremes and 20000 log-uniform positives.  A wrong
+    special-value select, or a q off by one for negative exponents (e.g. a
+    truncating division), fails here."""
+    rng = np.random.default_rng(5)
+    xs = [math.nan, 0.0, -0.0, math.inf, -math.inf, -1.0, -1e-300, 5e-324, 1e-310, 2.2250738585072014e-308,
+          1.7976931348623157e308, 1.0, 2.0, 0.5, 1e-9, 3e-5]
+    xs += [2.0 ** k for k in range(-1074, 1024, 7)]
+    xs += list(np.exp(rng.uniform(-700.0, 700.0, 20000)))
+    for x in xs:
+        got, want = kta.kta_pow_m02(float(x)), O.pow_m02(float(x))
+        if math.isnan(want):
+            assert math.isnan(got), x
+        else:
+            assert np.float64(got).tobytes() == np.float64(want).tobytes(), (x, got, want)
 
 
 def _p(a):
