@@ -149,37 +149,82 @@ def render(theta, renderer):
     return _KERR_RENDER.apply(theta, renderer)
 
 
+def _gauss_blur(img, sigma):
+    """Separable Gaussian blur (radius 3 sigma, edge pixels repeated) of an
+    [H, W, C] float64 image, differentiable through torch."""
+    torch = _torch()
+    F = torch.nn.functional
+    r = max(1, int(3.0 * sigma))
+    x = torch.arange(-r, r + 1, dtype=torch.float64, device=img.device)
+    w = torch.exp(-0.5 * (x / sigma) ** 2)
+    w = w / w.sum()
+    t = img.permute(2, 0, 1).unsqueeze(1)
+    t = F.conv2d(F.pad(t, (r, r, 0, 0), mode="replicate"), w.view(1, 1, 1, -1))
+    t = F.conv2d(F.pad(t, (0, 0, r, r), mode="replicate"), w.view(1, 1, -1, 1))
+    return t.squeeze(1).permute(1, 2, 0)
+
+
 def fit_adam(renderer, target_hdr, theta0, iters=200, step=(0.01, 0.01, 0.5, 0.01), hold=0.6, final_lr=0.1,
-             log_every=0):
+             blur_px=8.0, clip=3.0, betas=(0.9, 0.99), log_every=0):
     """Fit theta to a target frame with Adam on the L2 loss over the linear
     HDR colour channels.  Adam's step is about its learning rate in each
     coordinate whatever the gradient's scale, so theta is optimised as
     theta = step * z on z: `step` is the per-iteration step size in spin,
     inclination [rad], r_obs [M] and mdot.  The learning rate stays 1 for
     the first `hold` fraction of the iterations, then falls linearly to
-    `final_lr`.  The loss is only piecewise smooth (its frozen-event
-    gradient ignores the image's discontinuities, whose ripples make a
-    constant step wander), so the fit returns the iterate of lowest loss.
-    Returns (theta [4] list of the best iterate, history of (loss, theta)
-    before each update)."""
+    `final_lr`.
+
+    Coarse to fine: iteration k descends the L2 loss of both frames blurred
+    by a Gaussian of sigma = blur_px * (1 - k / (iters - 1)) pixels (the
+    plain L2 loss once sigma < 0.25 px, so the last iterations descend the
+    loss as stated).  The loss is only piecewise smooth: pixels that change
+    class (disk, shadow, sky) make it jump, and the frozen-event gradient
+    does not see those jumps; blurring spreads each jump over a few pixels
+    and damps it against the smooth part.
+
+    Spikes: a pixel whose orbit is chaotic in theta (near the photon ring)
+    now and then gives a frozen-event gradient ~100x the usual; one such
+    spike inflates Adam's second moment and the steps collapse for the
+    rest of the fit.  From the 6th iteration on, each gradient component
+    (of z) is clipped to `clip` times the median of its magnitudes over
+    the previous 10 iterations (clip=0: off), and the second moment's
+    decay is 0.99.
+
+    On the 512x512 scene, without blur and clipping the fit ends at
+    different points for starts 1e-7 apart (spin off by 0.02 to 0.24);
+    with them, starts 0, +-1e-7 and 2e-7 away end within 0.001 in spin,
+    0.02 deg, 0.02 M and 0.002 in mdot of the target, with the FP64 pair
+    and with the mixed forward + FP32 adjoint.  The iterate of lowest plain
+    L2 loss is returned.
+
+    Returns (theta [4] list of the best iterate, history of (plain L2 loss,
+    theta) before each update)."""
     torch = _torch()
     scale = torch.tensor(step, dtype=torch.float64)
     z = (torch.tensor([float(v) for v in theta0], dtype=torch.float64) / scale).requires_grad_(True)
-    opt = torch.optim.Adam([z], lr=1.0)
+    opt = torch.optim.Adam([z], lr=1.0, betas=betas)
     n_hold = int(hold * iters)
     target = target_hdr[..., :3].detach().to(torch.float64)
     history = []
+    grads = []  # per-iteration |gradient| of z, for the clipping
     best = (float("inf"), [float(v) for v in theta0])
     for it in range(iters):
         frac = 0.0 if it < n_hold else (it - n_hold) / max(iters - 1 - n_hold, 1)
         for group in opt.param_groups:
             group["lr"] = 1.0 + (final_lr - 1.0) * frac
+        sigma = blur_px * (1.0 - it / max(iters - 1, 1))
         opt.zero_grad()
         theta = z * scale
-        hdr = render(theta, renderer)
-        loss = ((hdr[..., :3].to(torch.float64) - target) ** 2).sum()
+        hdr = render(theta, renderer)[..., :3].to(torch.float64)
+        plain = ((hdr - target) ** 2).sum()
+        loss = ((_gauss_blur(hdr, sigma) - _gauss_blur(target, sigma)) ** 2).sum() if sigma >= 0.25 else plain
         loss.backward()
-        history.append((float(loss.detach()), [float(v) for v in theta.detach()]))
+        gabs = z.grad.detach().abs()
+        if clip > 0 and len(grads) >= 5:
+            lim = clip * torch.stack(grads[-10:]).median(dim=0).values
+            z.grad.copy_(torch.maximum(torch.minimum(z.grad, lim), -lim))
+        grads.append(gabs)
+        history.append((float(plain.detach()), [float(v) for v in theta.detach()]))
         if history[-1][0] < best[0]:
             best = history[-1]
         opt.step()
