@@ -26,7 +26,7 @@ def _single(torch, cfg, cam):
         return hdr.cpu().numpy(), cls.cpu().numpy(), r.stats()
 
 
-def _renders(n, side, precision=K.KT_PRECISION_FP64):
+def _renders(n, side, precision=K.KT_PRECISION_FP64, mixed_forms=False):
     out = []
     for k, (c, cam) in enumerate(K.batch_preset(limit=n, max_side=side)):
         # ragged sizes: make some renders non-square and not tile multiples
@@ -35,15 +35,23 @@ def _renders(n, side, precision=K.KT_PRECISION_FP64):
         if k % 3 == 2:
             c.height = max(7, c.height // 2 + 5)
         c.precision = precision
+        if mixed_forms and k % 2 == 1:
+            # the reference's as-written right-hand side and secant root
+            c.rhs_mode = K.KT_RHS_REFERENCE
+            c.disk_root_mode = K.KT_DISK_ROOT_SECANT_REF
         out.append((c, cam))
     return out
 
 
-@pytest.mark.parametrize("precision", [K.KT_PRECISION_FP64, K.KT_PRECISION_MIXED])
-def test_batch_matches_single_renders(torch_gpu, precision):
+@pytest.mark.parametrize("precision,mixed_forms", [(K.KT_PRECISION_FP64, False), (K.KT_PRECISION_MIXED, False),
+                                                   (K.KT_PRECISION_FP64, True), (K.KT_PRECISION_MIXED, True)])
+def test_batch_matches_single_renders(torch_gpu, precision, mixed_forms):
+    """mixed_forms: every other render uses the as-written right-hand side
+    and disk root; the batch kernel reads each render's form from its
+    record, a single render's kernel is compiled for its form."""
     from kerrtrace_amd.renderer import HipBatch
-    renders = _renders(8, 72, precision)
-    say(f"[batch] precision={precision} sizes={[(c.width, c.height) for c, _ in renders]}")
+    renders = _renders(8, 72, precision, mixed_forms)
+    say(f"[batch] precision={precision} mixed_forms={mixed_forms} sizes={[(c.width, c.height) for c, _ in renders]}")
     with HipBatch(renders) as b:
         hdr, cls = b.trace()
         st = b.stats()
