@@ -600,7 +600,8 @@ __device__ __forceinline__ bool uniform_flag(bool b) { return __builtin_amdgcn_r
 // claims as many items as there are idle lanes once `refill_threshold` of
 // them are idle (or all are), and ballot + popcount hand them out.  Items
 // past the end or outside their image leave the lane idle.  A new ray whose
-// step budget is empty is marked finished right away.
+// step budget is empty is marked finished right away.  Sets `drained` once
+// the claim reaches the end of the queue; the caller stops calling it then.
 template <bool BATCH>
 __device__ __forceinline__ void refill(Lane& L, Hot& H, const RenderParams* __restrict__ Ps, int n_renders,
                                        uint64_t total_items, int refill_threshold, DevCounters* __restrict__ counters,
