@@ -455,6 +455,83 @@ KT_HDI void rhs_vjp32(const V6f& Y, float pt, float a, float a2, float M, const 
     lam_a += ham_a.d;
 }
 
+// The exact right-hand side in FP32 (the primal of rhs_vjp32).  Outside
+// the horizon r > |a| and u = rho^2 - a^2 >= r^2 - a^2 > 0, so the radius
+// solve u + sqrt(u^2 + 4 a^2 y^2) does not cancel in FP32.
+KT_HDI V6f rhs_val32(const V6f& Y, float pt, float a, float a2, float M) {
+    const float x = Y.x, y = Y.y, z = Y.z, px = Y.px, py = Y.py, pz = Y.pz;
+    const float rho2 = fmaf(x, x, fmaf(y, y, z * z));
+    const float u = rho2 - a2;
+    const float ya = a2 * y;
+    const float s_sq = fmaf(u, u, 4.0f * (ya * y));
+    const float s = s_sq * rsq32(s_sq);
+    const float r2 = 0.5f * (u + s);
+    const float r = r2 * rsq32(r2);
+    const float denom = r2 + a2;
+    const float denomH = fmaf(r2, r2, ya * y);
+    const float inv_s = rcp32(s), inv_r = rcp32(r), inv_denom = rcp32(denom), inv_denomH = rcp32(denomH);
+    const float Nx = fmaf(r, x, -(a * z)), Nz = fmaf(r, z, a * x);
+    const float lx = Nx * inv_denom, ly = y * inv_r, lz = Nz * inv_denom;
+    const float r3 = r2 * r;
+    const float H = (M * r3) * inv_denomH;
+    const float Q = fmaf(lx, px, lz * pz);
+    const float lp = fmaf(ly, py, Q) - pt;
+    const float c = (2.0f * H) * lp;
+    const float inv_2r = 0.5f * inv_r;
+    const float w1 = fmaf(u, inv_s, 1.0f) * inv_2r;
+    const float w2 = fmaf(u + 2.0f * a2, inv_s, 1.0f) * inv_2r;
+    const float XP = fmaf(x, px, z * pz);
+    const float inv_r2 = inv_r * inv_r;
+    const float G = fmaf(fmaf(-2.0f * r, Q, XP), inv_denom, -((y * py) * inv_r2));
+    const float inv_denomH2 = inv_denomH * inv_denomH;
+    const float HR = ((M * r2) * inv_denomH2) * (3.0f * (ya * y) - r2 * r2);
+    const float HY = -(2.0f * M) * ya * (r3 * inv_denomH2);
+    const float lp2 = lp * lp;
+    const float Zz = fmaf(HR, lp2, c * G);
+    const float cD = c * inv_denom;
+    V6f d;
+    d.x = fmaf(-c, lx, px);
+    d.y = fmaf(-c, ly, py);
+    d.z = fmaf(-c, lz, pz);
+    d.px = fmaf(x * w1, Zz, cD * fmaf(r, px, a * pz));
+    d.py = fmaf(y * w2, Zz, fmaf(HY, lp2, (c * py) * inv_r));
+    d.pz = fmaf(z * w1, Zz, cD * fmaf(r, pz, -(a * px)));
+    return d;
+}
+
+// The inputs of stages 2..6 of a step from s, in FP32: the linearisation
+// points of the FP32 step reversal, within a few FP32 ulps of the forward
+// pass's FP64 stage inputs (the reversal's own FP32 rounding is larger).
+KT_HDI void dp_stages32(const V6f& s, float hs, float pt, float a, float a2, float M, V6f* Y) {
+#define KTA_F(M_) V6f{M_(x), M_(y), M_(z), M_(px), M_(py), M_(pz)}
+    const V6f k1 = rhs_val32(s, pt, a, a2, M);
+#define KTA_F2(c) fmaf(hs, (float)KTA_A21 * k1.c, s.c)
+    Y[0] = KTA_F(KTA_F2);
+    const V6f k2 = rhs_val32(Y[0], pt, a, a2, M);
+#define KTA_F3(c) fmaf(hs, fmaf((float)KTA_A32, k2.c, (float)KTA_A31 * k1.c), s.c)
+    Y[1] = KTA_F(KTA_F3);
+    const V6f k3 = rhs_val32(Y[1], pt, a, a2, M);
+#define KTA_F4(c) fmaf(hs, fmaf((float)KTA_A43, k3.c, fmaf((float)KTA_A42, k2.c, (float)KTA_A41 * k1.c)), s.c)
+    Y[2] = KTA_F(KTA_F4);
+    const V6f k4 = rhs_val32(Y[2], pt, a, a2, M);
+#define KTA_F5(c) \
+    fmaf(hs, fmaf((float)KTA_A54, k4.c, fmaf((float)KTA_A53, k3.c, fmaf((float)KTA_A52, k2.c, (float)KTA_A51 * k1.c))), s.c)
+    Y[3] = KTA_F(KTA_F5);
+    const V6f k5 = rhs_val32(Y[3], pt, a, a2, M);
+#define KTA_F6(c)                                                                                                  \
+    fmaf(hs,                                                                                                       \
+         fmaf((float)KTA_A65, k5.c,                                                                                \
+              fmaf((float)KTA_A64, k4.c, fmaf((float)KTA_A63, k3.c, fmaf((float)KTA_A62, k2.c, (float)KTA_A61 * k1.c)))), \
+         s.c)
+    Y[4] = KTA_F(KTA_F6);
+#undef KTA_F
+#undef KTA_F2
+#undef KTA_F3
+#undef KTA_F4
+#undef KTA_F5
+#undef KTA_F6
+}
+
 // dp_step_reverse in FP32 (same recursion; see dp_step_reverse).
 KT_HDI V6f dp_step_reverse32(const V6f& s, const V6f& ns, const V6f* Y, float hs, float pt, float a, float a2,
                              float M, V6f lam_ns, const V6f* lk, const V6f& lam_s_dir, float& lam_pt, float& lam_a) {

@@ -139,12 +139,14 @@ __device__ __forceinline__ void ray_adjoint(const RenderParams& P, const GradCon
             const V6 ns = sc.get6((uint32_t)u + 1 < len ? 6 * ((uint32_t)u + 1) : j_end);
             const double hs = ch[8 + u];
             const double Tr_n = ch[8 + K + u];
-            // the step's stage inputs, formed as the forward pass formed them
-            V6 Y[5];
-            (void)dp_step<MIXED>(sn, hs, pt, g, Y);
             const double r_now = ks_r(ns.x, ns.y, ns.z, P.aa);
             const bool horizon = r_now < P.rh_eps;
             const bool crossing = !horizon && ((sn.y < 0.0 && ns.y >= 0.0) || (sn.y > 0.0 && ns.y <= 0.0));
+            // the step's stage inputs, formed as the forward pass formed them
+            // (FP32 reversal of a step without a crossing: formed again in
+            // FP32, the points its FP32 VJPs linearise at)
+            V6 Y[5];
+            if (!F32 || crossing) (void)dp_step<MIXED>(sn, hs, pt, g, Y);
             V6 lk[6];
             V6 lam_s_dir = v6_zero();
             bool have_lk = false;
@@ -169,7 +171,11 @@ __device__ __forceinline__ void ray_adjoint(const RenderParams& P, const GradCon
             }
             if (F32) {
                 V6f Yf[5], lkf[6];
-                for (int i = 0; i < 5; ++i) Yf[i] = to_f(Y[i]);
+                if (crossing) {
+                    for (int i = 0; i < 5; ++i) Yf[i] = to_f(Y[i]);
+                } else {
+                    dp_stages32(to_f(sn), (float)hs, ptf, af, a2f, Mf, Yf);
+                }
                 if (have_lk)
                     for (int i = 0; i < 6; ++i) lkf[i] = to_f(lk[i]);
                 lamf = dp_step_reverse32(to_f(sn), to_f(ns), Yf, (float)hs, ptf, af, a2f, Mf, lamf,

@@ -137,6 +137,19 @@ void kta_shade(const double sp[13], const int ip[2], uint64_t seed_mix, const do
     vals[4] = o.lit ? 1.0 : 0.0;
 }
 
+// the FP32 stage inputs of a step (dp_stages32) next to the FP64 ones of dp_step
+void kta_stages32(const double s[6], double h, double pt, double a, double M, double y64[30], double y32[30]) {
+    const Geo g = make_geo(a, a * a, M, true);
+    V6 Y[5];
+    (void)dp_step(v6(s), h, pt, g, Y);
+    V6f Yf[5];
+    dp_stages32(to_f(v6(s)), (float)h, (float)pt, (float)a, (float)(a * a), (float)M, Yf);
+    for (int i = 0; i < 5; ++i) {
+        put(Y[i], y64 + 6 * i);
+        put(to_d(Yf[i]), y32 + 6 * i);
+    }
+}
+
 // the tracer's step-size controller factor x^(-1/5) (kt_math.hpp pow_m02)
 double kta_pow_m02(double x) { return pow_m02(x); }
 

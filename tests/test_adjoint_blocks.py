@@ -42,9 +42,32 @@ def kta():
     L.kta_crossing.argtypes = [_pd, _pd, _pd, d, i, _pd, _pd, _pd, _pd, _pd]
     L.kta_start.argtypes = [i, i, i, i, d, d, d, d, d, i, _pd, _pd]
     L.kta_shade.argtypes = [_pd, C.POINTER(C.c_int), C.c_uint64, _pd, _pd, _pd, _pd, d, d, i, _pd, _pd]
+    L.kta_stages32.argtypes = [_pd, d, d, d, d, _pd, _pd]
     L.kta_pow_m02.argtypes = [d]
     L.kta_pow_m02.restype = d
     return L
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.6, 0.998])
+def test_fp32_stage_inputs_match_fp64(kta, spin):
+    """The FP32 reversal's stage inputs, formed again in FP32 (dp_stages32:
+    FP32 right-hand side including the radius solve, FP32 stage sums), lie
+    within FP32 rounding of the FP64 stage inputs of the same step: the
+    linearisation points of the FP32 VJPs (measured 2.4e-6 of the state's
+    scale per component: FP32 rounding amplified by the tableau's weights,
+    up to 11.6 in magnitude).  A wrong tableau entry or a dropped term in
+    the FP32 right-hand side misses the bar (1e-5) by orders of magnitude."""
+    rng = np.random.default_rng(11)
+    y64, y32 = np.zeros(30), np.zeros(30)
+    worst = 0.0
+    for _ in range(200):
+        s, pt = _state(rng, spin)
+        h = float(rng.uniform(0.05, 2.0))
+        kta.kta_stages32(_p(s), h, pt, spin, 1.0, _p(y64), _p(y32))
+        Y64, Y32 = y64.reshape(5, 6), y32.reshape(5, 6)
+        scale = np.maximum(np.abs(Y64), 1.0)
+        worst = max(worst, float(np.max(np.abs(Y32 - Y64) / scale)))
+    assert worst <= 1e-5, worst
 
 
 def test_device_pow_m02_matches_oracle(kta):
