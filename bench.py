@@ -437,22 +437,50 @@ def copy_calibration(mib, reps=3):
     print(f"[bench] copied {mib} MiB x {reps} ({mib * 1024} KiB read and written per copy)", flush=True)
 
 
+TRACE_PRESETS = ("cpu256_a09", "hd1080_a0998", "uhd2160_i60", "grad512")
+
+
+def parse_trace_spec(spec):
+    """--trace-only's workload, preset[:tolerance[:fp64|mixed[:spin]]], e.g.
+    hd1080_a0998:1e-7:mixed or uhd2160_i60:1e-7:fp64:0.998, as a dict
+    (name, tolerance or None, precision or None, spin or None); exits with
+    a message on a malformed spec."""
+    parts = spec.split(":")
+    if not 1 <= len(parts) <= 4 or parts[0] not in TRACE_PRESETS:
+        sys.exit(f"--trace-only takes preset[:tolerance[:fp64|mixed[:spin]]] with a preset of {TRACE_PRESETS}")
+    out = {"name": parts[0], "tolerance": None, "precision": None, "spin": None}
+    try:
+        if len(parts) > 1 and parts[1]:
+            out["tolerance"] = float(parts[1])
+        if len(parts) > 3:
+            out["spin"] = float(parts[3])
+    except ValueError:
+        sys.exit("--trace-only: the tolerance and the spin are numbers")
+    if out["tolerance"] is not None and not out["tolerance"] > 0.0:
+        sys.exit("--trace-only: the tolerance must be positive")
+    if out["spin"] is not None and not -1.0 < out["spin"] < 1.0:
+        sys.exit("--trace-only: the spin must lie in (-1, 1)")
+    if len(parts) > 2:
+        if parts[2] not in ("fp64", "mixed"):
+            sys.exit("--trace-only: the precision is fp64 or mixed")
+        out["precision"] = parts[2]
+    return out
+
+
 def trace_only(spec, steps, launch_opts, lone=False):
-    """Trace one workload `steps` times and print its work counters: the
-    command profiles run under rocprofv3.  spec = preset[:tolerance[:fp64|mixed[:spin]]],
-    e.g. hd1080_a0998:1e-7:mixed or uhd2160_i60:1e-7:fp64:0.998.  lone:
-    trace only the frame's longest ray (lone_ray_scene)."""
+    """Trace one workload (parse_trace_spec) `steps` times and print its
+    work counters: the command profiles run under rocprofv3.  lone: trace
+    only the frame's longest ray (lone_ray_scene)."""
     import torch
 
     from kerrtrace_amd import config as K
     from kerrtrace_amd.renderer import HipRenderer
-    parts = spec.split(":")
-    kw = {"spin": float(parts[3])} if len(parts) > 3 else {}
-    cfg, cam = K.preset(parts[0], **kw)
-    if len(parts) > 1 and parts[1]:
-        cfg.tolerance = float(parts[1])
-    if len(parts) > 2:
-        cfg.precision = {"fp64": K.KT_PRECISION_FP64, "mixed": K.KT_PRECISION_MIXED}[parts[2]]
+    kw = {"spin": spec["spin"]} if spec["spin"] is not None else {}
+    cfg, cam = K.preset(spec["name"], **kw)
+    if spec["tolerance"] is not None:
+        cfg.tolerance = spec["tolerance"]
+    if spec["precision"] is not None:
+        cfg.precision = {"fp64": K.KT_PRECISION_FP64, "mixed": K.KT_PRECISION_MIXED}[spec["precision"]]
     if lone:
         cfg, cam = lone_ray_scene(cfg, cam, launch_opts)
         launch_opts = (1, 1, 1, 2, 256, 0)
@@ -539,13 +567,7 @@ def main():
     ap.add_argument("--lone-ray", action="store_true",
                     help="with --trace-only: trace only the frame's longest ray, alone (its critical path)")
     args = ap.parse_args()
-    if args.trace_only:
-        opts = [int(v) for v in args.launch_opts.split(",")] if args.launch_opts else None
-        trace_only(args.trace_only, args.steps, opts, lone=args.lone_ray)
-        return
-    if args.copy_mib > 0:
-        copy_calibration(args.copy_mib)
-        return
+    # every argument is checked before anything touches a GPU
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
                  "multi-GPU runs are not part of this build (use --gpus 1)")
@@ -561,6 +583,14 @@ def main():
         sys.exit("--steps must be at least 1 and --warmup at least 0")
     if not 0 <= args.streams <= 3:
         sys.exit("--streams takes 1, 2 or 3 (four streams would share a hardware queue)")
+    if args.lone_ray and not args.trace_only:
+        sys.exit("--lone-ray traces one workload's longest ray: it needs --trace-only")
+    if args.trace_only:
+        trace_only(parse_trace_spec(args.trace_only), args.steps, fixed_opts, lone=args.lone_ray)
+        return
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+        return
 
     import torch
 
