@@ -53,6 +53,27 @@ def test_fp64_frame_matches_oracle(torch_gpu, name, size):
     assert np.all(g_hdr[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("spin,noise,spp", [(-0.7, 1, 1), (0.0, 0, 2), (0.999, 1, 1)])
+def test_fp64_scene_variants_match_oracle_bitwise(torch_gpu, spin, noise, spp):
+    """A retrograde spin (disk from the retrograde ISCO, negative frame
+    term), a Schwarzschild hole with the noise off and two samples per
+    pixel, and a near-extremal spin: every hit class and HDR float equal to
+    the oracle's."""
+    cfg, cam = K.scaled(*K.preset("cpu256_a09"), 96, 80)
+    cfg.spin = spin
+    cfg.noise_enabled = noise
+    cfg.spp = spp
+    say(f"[parity] variant spin={spin} noise={noise} spp={spp}")
+    g_hdr, g_cls, st = _render_gpu(torch_gpu, cfg, cam)
+    o_hdr, o_cls, _, ost = O.render(cfg, cam)
+    m = compare(g_hdr, g_cls, o_hdr, o_cls)
+    say(f"[parity] variant: {m}; attempts gpu {st['attempts']} oracle {ost.attempts}")
+    assert np.array_equal(g_cls, o_cls)
+    # every HDR float equal to the oracle's FP64 value rounded to FP32
+    assert np.array_equal(g_hdr[..., :3], o_hdr.astype(np.float32))
+    assert int(st["attempts"]) == int(ost.attempts)
+
+
 @pytest.mark.parametrize("name,spin", [("hd1080_a0998", None), ("uhd2160_i60", 0.998)])
 def test_fp64_full_size_frame_matches_oracle(torch_gpu, name, spin):
     """The measured frames at their full size (1920x1080 at tol 1e-9;
