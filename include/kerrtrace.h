@@ -260,11 +260,12 @@ kt_status kt_trace_ex(kt_renderer* r, const kt_camera* cam, float* hdr,
 kt_status kt_set_launch_opts(kt_renderer* r, const kt_launch_opts* opts);
 /* Waits for the last issued trace and returns its counters. */
 kt_status kt_get_trace_stats(kt_renderer* r, kt_trace_stats* out);
-/* bloom: float4[W*H] scratch; ldr: uint32[W*H] 0x00RRGGBB. */
+/* bloom: float4[W*H] scratch (the passes use its first 12*W*H bytes as
+ * packed RGB); ldr: uint32[W*H] 0x00RRGGBB. */
 kt_status kt_post_process(kt_renderer* r, const float* hdr, float* bloom,
                           uint32_t* ldr, void* stream);
 /* As kt_post_process, with the bloom intermediate in a device array the
- * library holds itself (32 MiB per device, kept until the process ends)
+ * library holds itself (24 MiB per device, kept until the process ends)
  * instead of a caller buffer.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops).  It calls no allocator.  Frames of up to 2,097,152

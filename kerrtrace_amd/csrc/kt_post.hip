@@ -5,7 +5,9 @@
 //     horizontal truncated Gaussian with weights renormalised over the
 //     in-bounds taps (kernel.cu:54-84, 144-173).  A block owns a row
 //     segment and stages segment + 2R halo pixels (rgb) in LDS once; each
-//     thread computes kHK adjacent output pixels.
+//     thread computes kHK adjacent output pixels.  The intermediate is
+//     packed RGB (3 floats, 12 bytes per pixel: the reference's float4
+//     alpha is never read), in the caller's float4-sized bloom buffer.
 //  2. post_v: vertical pass of the same Gaussian over bloom_h's output,
 //     hdr + intensity * blur, ACES fit, sRGB OETF, round-to-nearest-even
 //     bytes packed as 0x00RRGGBB (kernel.cu:175-251).  A block owns a
@@ -74,7 +76,7 @@ __device__ __forceinline__ void fir_tile(const Load& ld, const float* __restrict
 }
 
 // ws_col: [W] sums of the in-bounds horizontal weights of each column.
-__global__ __launch_bounds__(kHThreads) void bloom_h(const float4* __restrict__ hdr, float4* __restrict__ bloom, int W,
+__global__ __launch_bounds__(kHThreads) void bloom_h(const float4* __restrict__ hdr, float* __restrict__ bloom, int W,
                                                      int R, const float* __restrict__ wts,
                                                      const float* __restrict__ ws_col, float threshold) {
     extern __shared__ float lds[];  // (kHThreads * kHK + 2R) * 3 floats
@@ -112,13 +114,15 @@ __global__ __launch_bounds__(kHThreads) void bloom_h(const float4* __restrict__ 
             v[2] = base[m * 3 + 2];
         },
         wts, 2 * R + 1, acc);
-    float4* out = bloom + (size_t)y * W;
+    float* out = bloom + (size_t)y * W * 3;
 #pragma unroll
     for (int k = 0; k < kHK; ++k) {
         const int x = xb + k;
         if (x < W) {
             const float ws = ws_col[x];
-            out[x] = make_float4(acc[k][0] / ws, acc[k][1] / ws, acc[k][2] / ws, 0.f);
+            out[x * 3 + 0] = acc[k][0] / ws;
+            out[x * 3 + 1] = acc[k][1] / ws;
+            out[x * 3 + 2] = acc[k][2] / ws;
         }
     }
 }
@@ -155,7 +159,7 @@ __device__ __forceinline__ uint32_t display(float fx, float fy, float fz) {
 // of the block's TH = nw K rows.  ws_row: [H] sums of the in-bounds
 // vertical weights of each row.
 template <int K>
-__global__ __launch_bounds__(512) void post_v(const float4* __restrict__ hdr, const float4* __restrict__ bloom,
+__global__ __launch_bounds__(512) void post_v(const float4* __restrict__ hdr, const float* __restrict__ bloom,
                                               uint32_t* __restrict__ out, int W, int H, int R,
                                               const float* __restrict__ wts, const float* __restrict__ ws_row,
                                               float intensity) {
@@ -169,12 +173,17 @@ __global__ __launch_bounds__(512) void post_v(const float4* __restrict__ hdr, co
     const int rows = TH + 2 * R;
     for (int r = wv; r < rows; r += nw) {
         int sy = y0 - R + r;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < W && sy >= 0 && sy < H) v = bloom[(size_t)sy * W + x];
+        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+        if (x < W && sy >= 0 && sy < H) {
+            const float* b = bloom + ((size_t)sy * W + x) * 3;
+            v0 = b[0];
+            v1 = b[1];
+            v2 = b[2];
+        }
         float* d = lds + ((size_t)r * kColTile + tx) * 3;
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
+        d[0] = v0;
+        d[1] = v1;
+        d[2] = v2;
     }
     __syncthreads();
     const int yb = y0 + wv * K;
@@ -236,8 +245,8 @@ bool post_v_plan(int R, size_t max_lds, int* k, int* waves) {
 // frame before post_v reads it, so nothing is cleared between calls.  In
 // host code `scratch` names a host-side stand-in: kernels get the device
 // address that post_scratch_address resolves per device.
-static constexpr size_t kNoAllocMaxPixels = size_t(1) << 21;  // 2,097,152 px: 32 MiB of float4
-static __device__ __align__(256) float4 scratch[kNoAllocMaxPixels];
+static constexpr size_t kNoAllocMaxPixels = size_t(1) << 21;  // 2,097,152 px of packed RGB: 24 MiB
+static __device__ __align__(256) float scratch[3 * kNoAllocMaxPixels];
 
 static constexpr int kMaxDevices = 64;
 static std::mutex g_scratch_mu;
@@ -266,7 +275,7 @@ hipError_t launch_bloom_h(const float* hdr, float* bloom, int W, int H, int R, c
                           const float* d_ws_col, float threshold, hipStream_t stream) {
     dim3 grid((W + kHThreads * kHK - 1) / (kHThreads * kHK), H);
     hipLaunchKernelGGL(bloom_h, grid, dim3(kHThreads), bloom_h_lds_bytes(R), stream,
-                       reinterpret_cast<const float4*>(hdr), reinterpret_cast<float4*>(bloom), W, R, d_wts, d_ws_col,
+                       reinterpret_cast<const float4*>(hdr), bloom, W, R, d_wts, d_ws_col,
                        threshold);
     return hipGetLastError();
 }
@@ -284,12 +293,11 @@ hipError_t launch_post_v(const float* hdr, const float* bloom, uint32_t* out, in
     dim3 grid((W + kColTile - 1) / kColTile, (H + TH - 1) / TH);
     const size_t lds = post_v_lds_bytes(R, TH);
     const float4* h4 = reinterpret_cast<const float4*>(hdr);
-    const float4* b4 = reinterpret_cast<const float4*>(bloom);
     if (K == 8)
-        hipLaunchKernelGGL(post_v<8>, grid, dim3(64 * waves), lds, stream, h4, b4, out, W, H, R, d_wts, d_ws_row,
+        hipLaunchKernelGGL(post_v<8>, grid, dim3(64 * waves), lds, stream, h4, bloom, out, W, H, R, d_wts, d_ws_row,
                            intensity);
     else
-        hipLaunchKernelGGL(post_v<1>, grid, dim3(64 * waves), lds, stream, h4, b4, out, W, H, R, d_wts, d_ws_row,
+        hipLaunchKernelGGL(post_v<1>, grid, dim3(64 * waves), lds, stream, h4, bloom, out, W, H, R, d_wts, d_ws_row,
                            intensity);
     return hipGetLastError();
 }
