@@ -42,18 +42,19 @@ namespace kt {
 // the exact form's unit factor and the as-written form's 1/2 fold away.
 enum : int { kRhsPerRender = 0, kRhsExact = 1, kRhsAsWritten = 2 };
 
-// The DP5(4) stage, solution, error and dense-output coefficients of an
-// attempt, read from constant memory on every loop turn (scalar loads,
-// several coefficients per load) instead of being formed from 64-bit
-// immediates: the kernel has no scalar registers to keep them across loop
-// turns, and forming 22 of them took 44 s_mov_b32 per attempt, issue
+// The DP5(4) coefficients from stage 5 on (stage, solution, error and
+// dense-output weights), read from constant memory on every loop turn
+// (scalar loads, eight coefficients per load) instead of being formed from
+// 64-bit immediates: the kernel has no scalar registers to keep them across
+// loop turns, and forming them took two s_mov_b32 each per attempt, issue
 // slots on the serial chain of the wave that holds the frame's longest ray.
+// Stages 2 to 4 keep their few immediates, so the loads issued at the top
+// of an attempt arrive while those stages run instead of stalling it.
 struct Dp5Coef {
-    double a21, a31, a32, a41, a42, a43, a51, a52, a53, a54, a61, a62, a63, a64, a65;
+    double a51, a52, a53, a54, a61, a62, a63, a64, a65;
     double b1, b3, b4, b5, b6, e1, e3, e4, e5, e6, e7, d1, d3, d4, d5, d6, d7;
 };
-static __constant__ Dp5Coef kDp5 = {KT_A21, KT_A31, KT_A32, KT_A41, KT_A42, KT_A43, KT_A51, KT_A52,
-                                    KT_A53, KT_A54, KT_A61, KT_A62, KT_A63, KT_A64, KT_A65,
+static __constant__ Dp5Coef kDp5 = {KT_A51, KT_A52, KT_A53, KT_A54, KT_A61, KT_A62, KT_A63, KT_A64, KT_A65,
                                     KT_B1, KT_B3, KT_B4, KT_B5, KT_B6, KT_E1, KT_E3, KT_E4, KT_E5,
                                     KT_E6, KT_E7, Dense::d1, Dense::d3, Dense::d4, Dense::d5,
                                     Dense::d6, Dense::d7};
@@ -138,9 +139,9 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
             {
                 const V6 k1 = L.k1;
                 // stage inputs s + h * sum_j a_ij k_j as fma chains
-#define KT_ST2(c) fma(hs, C->a21 * k1.c, s.c)
-#define KT_ST3(c) fma(hs, fma(C->a32, k2.c, C->a31 * k1.c), s.c)
-#define KT_ST4(c) fma(hs, fma(C->a43, k3.c, fma(C->a42, k2.c, C->a41 * k1.c)), s.c)
+#define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
+#define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
+#define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
 #define KT_ST5(c) fma(hs, fma(C->a54, k4.c, fma(C->a53, k3.c, fma(C->a52, k2.c, C->a51 * k1.c))), s.c)
 #define KT_ST6(c) \
         fma(hs, fma(C->a65, k5.c, fma(C->a64, k4.c, fma(C->a63, k3.c, fma(C->a62, k2.c, C->a61 * k1.c)))), s.c)
