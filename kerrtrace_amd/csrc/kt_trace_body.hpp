@@ -43,18 +43,24 @@ namespace kt {
 enum : int { kRhsPerRender = 0, kRhsExact = 1, kRhsAsWritten = 2 };
 
 // The DP5(4) coefficients from stage 5 on (stage, solution, error and
-// dense-output weights), read from constant memory on every loop turn
-// (scalar loads, eight coefficients per load) instead of being formed from
-// 64-bit immediates: the kernel has no scalar registers to keep them across
-// loop turns, and forming them took two s_mov_b32 each per attempt, issue
-// slots on the serial chain of the wave that holds the frame's longest ray.
-// Stages 2 to 4 keep their few immediates, so the loads issued at the top
-// of an attempt arrive while those stages run instead of stalling it.
+// dense-output weights).  The FP64 tracer reads them from constant memory
+// on every loop turn (scalar loads, eight coefficients per load) instead of
+// forming them from 64-bit immediates: the kernel has no scalar registers
+// to keep them across loop turns, and forming them took two s_mov_b32 each
+// per attempt, issue slots on the serial chain of the wave that holds the
+// frame's longest ray.  Stages 2 to 4 keep their few immediates, so the
+// loads issued at the top of an attempt arrive while those stages run
+// instead of stalling it.  The mixed tracer keeps immediates throughout
+// (measured: the loads made its 3840x2160 frames 1.3% slower).
 struct Dp5Coef {
     double a51, a52, a53, a54, a61, a62, a63, a64, a65;
     double b1, b3, b4, b5, b6, e1, e3, e4, e5, e6, e7, d1, d3, d4, d5, d6, d7;
 };
 static __constant__ Dp5Coef kDp5 = {KT_A51, KT_A52, KT_A53, KT_A54, KT_A61, KT_A62, KT_A63, KT_A64, KT_A65,
+                                    KT_B1, KT_B3, KT_B4, KT_B5, KT_B6, KT_E1, KT_E3, KT_E4, KT_E5,
+                                    KT_E6, KT_E7, Dense::d1, Dense::d3, Dense::d4, Dense::d5,
+                                    Dense::d6, Dense::d7};
+static constexpr Dp5Coef kDp5Imm = {KT_A51, KT_A52, KT_A53, KT_A54, KT_A61, KT_A62, KT_A63, KT_A64, KT_A65,
                                     KT_B1, KT_B3, KT_B4, KT_B5, KT_B6, KT_E1, KT_E3, KT_E4, KT_E5,
                                     KT_E6, KT_E7, Dense::d1, Dense::d3, Dense::d4, Dense::d5,
                                     Dense::d6, Dense::d7};
@@ -133,7 +139,8 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
             // its loads are issued inside the attempt, not hoisted out of
             // the loop into registers it does not have
             Dp5Ptr C = (Dp5Ptr)&kDp5;
-            asm volatile("" : "+s"(C));
+            if (!MIXED) asm volatile("" : "+s"(C));
+#define KT_C(f) (MIXED ? kDp5Imm.f : C->f)
             V6 ns, eacc, dacc;
             const V6 k1s = L.k1;
             {
@@ -142,9 +149,9 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
 #define KT_ST2(c) fma(hs, KT_A21 * k1.c, s.c)
 #define KT_ST3(c) fma(hs, fma(KT_A32, k2.c, KT_A31 * k1.c), s.c)
 #define KT_ST4(c) fma(hs, fma(KT_A43, k3.c, fma(KT_A42, k2.c, KT_A41 * k1.c)), s.c)
-#define KT_ST5(c) fma(hs, fma(C->a54, k4.c, fma(C->a53, k3.c, fma(C->a52, k2.c, C->a51 * k1.c))), s.c)
+#define KT_ST5(c) fma(hs, fma(KT_C(a54), k4.c, fma(KT_C(a53), k3.c, fma(KT_C(a52), k2.c, KT_C(a51) * k1.c))), s.c)
 #define KT_ST6(c) \
-        fma(hs, fma(C->a65, k5.c, fma(C->a64, k4.c, fma(C->a63, k3.c, fma(C->a62, k2.c, C->a61 * k1.c)))), s.c)
+        fma(hs, fma(KT_C(a65), k5.c, fma(KT_C(a64), k4.c, fma(KT_C(a63), k3.c, fma(KT_C(a62), k2.c, KT_C(a61) * k1.c)))), s.c)
 #define KT_STAGE(M) V6{M(x), M(y), M(z), M(px), M(py), M(pz)}
                 const V6 k2 = KT_RHS(KT_STAGE(KT_ST2));
                 const V6 k3 = KT_RHS(KT_STAGE(KT_ST3));
@@ -154,19 +161,19 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
                 // The 5th-order solution b.k and the error (b - bhat).k are fma
                 // chains in stage order; their leading terms are folded now so
                 // k2..k5 are dead while stage 6 and 7 are evaluated.
-#define KT_B4ACC(c) fma(C->b5, k5.c, fma(C->b4, k4.c, fma(C->b3, k3.c, C->b1 * k1.c)))
-#define KT_E5ACC(c) fma(C->e5, k5.c, fma(C->e4, k4.c, fma(C->e3, k3.c, C->e1 * k1.c)))
+#define KT_B4ACC(c) fma(KT_C(b5), k5.c, fma(KT_C(b4), k4.c, fma(KT_C(b3), k3.c, KT_C(b1) * k1.c)))
+#define KT_E5ACC(c) fma(KT_C(e5), k5.c, fma(KT_C(e4), k4.c, fma(KT_C(e3), k3.c, KT_C(e1) * k1.c)))
                 V6 bacc = KT_STAGE(KT_B4ACC);
                 eacc = KT_STAGE(KT_E5ACC);
                 // dense-output sum d1 k1 + d3 k3 + ... + d7 k7 as an fma chain (dense_sum)
-#define KT_D5ACC(c) fma(C->d5, k5.c, fma(C->d4, k4.c, fma(C->d3, k3.c, C->d1 * k1.c)))
+#define KT_D5ACC(c) fma(KT_C(d5), k5.c, fma(KT_C(d4), k4.c, fma(KT_C(d3), k3.c, KT_C(d1) * k1.c)))
                 dacc = KT_STAGE(KT_D5ACC);
                 const V6 k6 = KT_RHS(in6);
-#define KT_NS(c) fma(hs, fma(C->b6, k6.c, bacc.c), s.c)
-#define KT_EACC6(c) fma(C->e6, k6.c, eacc.c)
+#define KT_NS(c) fma(hs, fma(KT_C(b6), k6.c, bacc.c), s.c)
+#define KT_EACC6(c) fma(KT_C(e6), k6.c, eacc.c)
                 ns = KT_STAGE(KT_NS);  // = the 7th stage input (a7j = b_j)
                 eacc = KT_STAGE(KT_EACC6);
-#define KT_D6ACC(c) fma(C->d6, k6.c, dacc.c)
+#define KT_D6ACC(c) fma(KT_C(d6), k6.c, dacc.c)
                 dacc = KT_STAGE(KT_D6ACC);
 #undef KT_ST2
 #undef KT_ST3
@@ -187,12 +194,12 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
             if (MIXED) r_ns = ks_r(ns.x, ns.y, ns.z, g.aa);
 #undef KT_RHS
             cnt.rhs += 6;
-#define KT_D7ACC(c) fma(C->d7, k7.c, dacc.c)
+#define KT_D7ACC(c) fma(KT_C(d7), k7.c, dacc.c)
             dacc = KT_STAGE(KT_D7ACC);
 #undef KT_D7ACC
             // embedded error, max norm relative to the start state (kerr_trace.cuh:104-116)
             const double tol = L.tol;
-#define KT_ERR(c) div_pos(fabs(hs * fma(C->e7, k7.c, eacc.c)), tol * fmax(fabs(s.c), 1.0))
+#define KT_ERR(c) div_pos(fabs(hs * fma(KT_C(e7), k7.c, eacc.c)), tol * fmax(fabs(s.c), 1.0))
             double ex = KT_ERR(x), ey = KT_ERR(y), ez = KT_ERR(z);
             double epx = KT_ERR(px), epy = KT_ERR(py), epz = KT_ERR(pz);
 #undef KT_ERR
@@ -200,10 +207,11 @@ __device__ __forceinline__ void trace_loop(const RenderParams* __restrict__ Ps, 
             double err = fmax(fmax(fmax(ex, ey), fmax(ez, epx)), fmax(epy, epz));
             if (__builtin_expect(!(err <= 1.79769313486231570e308), 0)) {
                 // a non-finite step error: the library quotients (special values)
-#define KT_ERRX(c) (fabs(hs * fma(C->e7, k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
+#define KT_ERRX(c) (fabs(hs * fma(KT_C(e7), k7.c, eacc.c)) / (tol * fmax(fabs(s.c), 1.0)))
                 err = fmax(fmax(fmax(KT_ERRX(x), KT_ERRX(y)), fmax(KT_ERRX(z), KT_ERRX(px))),
                            fmax(KT_ERRX(py), KT_ERRX(pz)));
 #undef KT_ERRX
+#undef KT_C
             }
             cnt.attempts++;
             L.ray_att++;
