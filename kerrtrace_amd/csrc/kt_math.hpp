@@ -145,20 +145,25 @@ KT_HDI Geo make_geo(double a, double aa, double M, bool exact) {
 // whose u/s term is halved, enters through w2).  Products that feed a sum
 // are fused with explicit fma; the reciprocals 1/s, 1/r, 1/(r^2+a^2),
 // 1/(r^4+a^2y^2) come from one division.  r_out (optional) receives the
-// Kerr-Schild radius of st, the same value ks_r() returns.
+// Kerr-Schild radius of st, the same value ks_r() returns.  Two products are
+// formed in an order that rounds identically to the oracle's get_derivs
+// (scalings by 2 and 4 are exact, so they commute with rounding): 4 a^2y y
+// as 4 (a^2y y), sharing a^2y y with r^4 + a^2y^2, and 2 H (l.p - pt) as
+// ((2M r^3) / (r^4 + a^2y^2)) (l.p - pt), which drops the separate 2 H.
 KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
     const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
     const double a2 = g.aa, a = g.a;
     double rho2 = fma(x, x, fma(y, y, z * z));
     double u = rho2 - a2;
     double ya = a2 * y;
-    double s_sq = fma(u, u, 4.0 * ya * y);
+    double yay = ya * y;
+    double s_sq = fma(u, u, 4.0 * yay);
     double s = sqrt_pos(s_sq);
     double r2 = 0.5 * (u + s);
     double r = sqrt_pos(r2);
     if (r_out) *r_out = r;
     double denom = r2 + a2;
-    double denomH = fma(r2, r2, ya * y);
+    double denomH = fma(r2, r2, yay);
     double sr = s * r;
     double dd = denom * denomH;
     double q = div_pos(1.0, sr * dd);
@@ -172,9 +177,9 @@ KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
     double ly = y * inv_r;
     double lz = Nz * inv_denom;
     double r3 = r2 * r;
-    double H = (g.M * r3) * inv_denomH;
+    double H2 = ((2.0 * g.M) * r3) * inv_denomH;  // 2 H
     double lp = fma(lx, px, fma(ly, py, fma(lz, pz, -pt)));
-    double c = (2.0 * H) * lp;
+    double c = H2 * lp;
     V6 d;
     d.x = fma(-c, lx, px);
     d.y = fma(-c, ly, py);
@@ -242,10 +247,10 @@ __device__ __forceinline__ V6 rhs32(const V6& st, float ptf, const Geo& g, const
     const float ly = yf * inv_r;
     const float lz = fmaf(r, zf, a * xf) * inv_denom;
     const float r3 = r2 * r;
-    const float H = (M * r3) * inv_denomH;
+    const float H2 = ((2.0f * M) * r3) * inv_denomH;  // 2 H, rounded as (2 H) would be
     const float Q = fmaf(lx, pxf, lz * pzf);
     const float lp = fmaf(ly, pyf, Q) - ptf;
-    const float c = (2.0f * H) * lp;
+    const float c = H2 * lp;
     const float inv_2r = 0.5f * inv_r;
     const float w1 = fmaf(uf, inv_s, 1.0f) * inv_2r;
     const float w2 = fmaf(uf + gf.w2c, gf.w2f * inv_s, 1.0f) * inv_2r;
