@@ -145,11 +145,10 @@ KT_HDI Geo make_geo(double a, double aa, double M, bool exact) {
 // whose u/s term is halved, enters through w2).  Products that feed a sum
 // are fused with explicit fma; the reciprocals 1/s, 1/r, 1/(r^2+a^2),
 // 1/(r^4+a^2y^2) come from one division.  r_out (optional) receives the
-// Kerr-Schild radius of st, the same value ks_r() returns.  Two products are
-// formed in an order that rounds identically to the oracle's get_derivs
-// (scalings by 2 and 4 are exact, so they commute with rounding): 4 a^2y y
-// as 4 (a^2y y), sharing a^2y y with r^4 + a^2y^2, and 2 H (l.p - pt) as
-// ((2M r^3) / (r^4 + a^2y^2)) (l.p - pt), which drops the separate 2 H.
+// Kerr-Schild radius of st, the same value ks_r() returns.  a^2y y is formed
+// once, for 4 a^2y^2 and for r^4 + a^2y^2, and 2 H as (2M r^3)/(r^4 + a^2y^2)
+// (scalings by 2 and 4 are exact, so these round as (4 a^2y) y and
+// 2 (M r^3/(r^4 + a^2y^2)) would).
 KT_HDI V6 rhs(const V6& st, double pt, const Geo& g, double* r_out = nullptr) {
     const double x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
     const double a2 = g.aa, a = g.a;

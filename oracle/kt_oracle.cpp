@@ -312,7 +312,9 @@ void ks_bl(S x, S y, S z, const Params<S>& p, S& r, S& sin_th, S& cos_th, S& phi
 // tracer repeats it operation for operation): products that feed a sum are
 // fused with explicit fma_, and the four reciprocals 1/s, 1/r,
 // 1/(r^2+a^2), 1/(r^4+a^2y^2) come from one division
-// q = 1/(s r (r^2+a^2)(r^4+a^2y^2)).
+// q = 1/(s r (r^2+a^2)(r^4+a^2y^2)); a^2y y is formed once (for 4a^2y^2
+// and r^4 + a^2y^2) and 2H as (2M r^3)/(r^4+a^2y^2), which in binary64
+// round exactly as (4 a^2y) y and 2 (M r^3/(r^4+a^2y^2)) would.
 template <class S>
 State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     const S x = st.x, y = st.y, z = st.z, px = st.px, py = st.py, pz = st.pz;
@@ -320,12 +322,13 @@ State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     S rho2 = fma_(x, x, fma_(y, y, z * z));
     S u = rho2 - a2;
     S ya = a2 * y;
-    S s_sq = fma_(u, u, 4.0 * ya * y);
+    S yay = ya * y;
+    S s_sq = fma_(u, u, 4.0 * yay);
     S s = sqrt_(s_sq);
     S r2 = 0.5 * (u + s);
     S r = sqrt_(r2);
     S denom = r2 + a2;
-    S denomH = fma_(r2, r2, ya * y);
+    S denomH = fma_(r2, r2, yay);
     S sr = s * r;
     S dd = denom * denomH;
     S q = 1.0 / (sr * dd);
@@ -339,9 +342,10 @@ State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     S ly = y * inv_r;
     S lz = Nz * inv_denom;
     S r3 = r2 * r;
-    S H = (p.M * r3) * inv_denomH;
+    S M2 = 2.0 * p.M;
+    S H2 = (M2 * r3) * inv_denomH;  // 2H
     S lp = fma_(lx, px, fma_(ly, py, fma_(lz, pz, -pt)));
-    S c = (2.0 * H) * lp;
+    S c = H2 * lp;
     State<S> d;
     d.x = fma_(-c, lx, px);
     d.y = fma_(-c, ly, py);
@@ -355,7 +359,7 @@ State<S> get_derivs(const State<S>& st, S pt, const Params<S>& p) {
     S G = fma_(fma_(-2.0 * r, Q, XP), inv_denom, -((y * py) * inv_r2));
     S inv_denomH2 = inv_denomH * inv_denomH;
     S HR = ((p.M * r2) * inv_denomH2) * fma_(3.0 * ya, y, -(r2 * r2));
-    S HY = -((2.0 * p.M) * ya) * (r3 * inv_denomH2);
+    S HY = -(M2 * ya) * (r3 * inv_denomH2);
     S lp2 = lp * lp;
     S Z = fma_(HR, lp2, c * G);
     S cD = c * inv_denom;
