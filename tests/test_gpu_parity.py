@@ -74,12 +74,14 @@ def test_fp64_scene_variants_match_oracle_bitwise(torch_gpu, spin, noise, spp):
     assert int(st["attempts"]) == int(ost.attempts)
 
 
-@pytest.mark.parametrize("name,spin", [("hd1080_a0998", None), ("uhd2160_i60", 0.998)])
+@pytest.mark.parametrize("name,spin", [("hd1080_a0998", None), ("uhd2160_i60", 0.0), ("uhd2160_i60", 0.5),
+                                       ("uhd2160_i60", 0.9), ("uhd2160_i60", 0.998)])
 def test_fp64_full_size_frame_matches_oracle(torch_gpu, name, spin):
     """The measured frames at their full size (1920x1080 at tol 1e-9;
-    3840x2160 at i = 60 deg, tol 1e-7, in FP64): the oracle finishes them in
-    seconds on the host's threads.  The frames are the oracle's bit for bit:
-    every hit class, every attempt count and every HDR float."""
+    3840x2160 at i = 60 deg, tol 1e-7, in FP64, at the four spins the mixed
+    tracer is compared at): the oracle finishes them in seconds on the host's
+    threads.  The frames are the oracle's bit for bit: every hit class, every
+    attempt count and every HDR float."""
     cfg, cam = K.preset(name) if spin is None else K.preset(name, spin=spin)
     cfg.precision = K.KT_PRECISION_FP64
     say(f"[parity] fp64 full size {name} {cfg.width}x{cfg.height}")
