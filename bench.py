@@ -590,7 +590,7 @@ def main():
     # every argument is checked before anything touches a GPU
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: rays are independent, so the frame shards with no exchange; "
-                 "multi-GPU runs are not part of this build (use --gpus 1)")
+                 "the renderer traces on one GPU (use --gpus 1)")
     fixed_opts = None
     if args.launch_opts:
         try:

@@ -1,7 +1,7 @@
 // FP64 CPU oracle of KerrTrace's per-pixel geodesic renderer.
 //
-// TEST INFRASTRUCTURE ONLY: this library is the checker that the product's
-// HIP kernels are compared against.  Only tests, the smoke check and the
+// TEST INFRASTRUCTURE ONLY: this library is the reference the product's
+// HIP kernels are compared against in tests.  Only tests, the smoke check and the
 // benchmark's CPU-baseline leg load it; the product never links or calls it.
 //
 // It restates the reference algorithm in double precision, one function per

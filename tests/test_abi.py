@@ -167,7 +167,7 @@ def test_null_arguments_return_errors():
 def test_buffer_guard_rejects_mismatched_tensors():
     """Caller buffers reach the kernels as raw pointers, so the Python host
     refuses any tensor that is not the exact contiguous device array a
-    kernel indexes (CPU tensors stand in here; on a GPU box the same guard
+    kernel indexes (CPU tensors stand in here; with a GPU the same guard
     runs on device tensors)."""
     import torch
 
