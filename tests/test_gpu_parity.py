@@ -35,6 +35,9 @@ FP64_MAX_REL = 1e-5
 
 @pytest.mark.parametrize("name,size", [("cpu256_a09", None), ("hd1080_a0998", (480, 270))])
 def test_fp64_frame_matches_oracle(torch_gpu, name, size):
+    """The 256x256 workload at its full size and the 1080p scene at 480x270:
+    within the parity bar, and in fact bit for bit the oracle's frame (every
+    hit class and HDR float, every work counter)."""
     cfg, cam = K.preset(name)
     if size is not None:
         cfg, cam = K.scaled(cfg, cam, *size)
@@ -46,10 +49,9 @@ def test_fp64_frame_matches_oracle(torch_gpu, name, size):
     assert m["class_mismatch_non_boundary"] == 0
     assert m["class_mismatch_frac"] <= FP64_MAX_MISMATCH_FRAC
     assert m["max_rel"] <= FP64_MAX_REL
-    # identical trajectories give identical work counters (allowing for a
-    # shading-only difference that moves a transmittance cutoff)
-    assert abs(int(st["attempts"]) - int(ost.attempts)) <= 1e-4 * ost.attempts
-    assert abs(int(st["crossings"]) - int(ost.crossings)) <= 1e-4 * max(ost.crossings, 1)
+    assert np.array_equal(g_cls, o_cls)
+    assert np.array_equal(g_hdr[..., :3], o_hdr.astype(np.float32))
+    assert int(st["attempts"]) == int(ost.attempts) and int(st["crossings"]) == int(ost.crossings)
     assert np.all(g_hdr[..., 3] == 1.0)
 
 
