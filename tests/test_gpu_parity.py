@@ -112,6 +112,10 @@ def test_secant_reference_root_mode(torch_gpu):
     say(f"[parity] secant: {m}")
     assert m["class_mismatch_non_boundary"] == 0
     assert m["max_rel"] <= FP64_MAX_REL
+    # bit for bit: every class, HDR float and attempt
+    assert np.array_equal(g_cls, o_cls)
+    assert np.array_equal(g_hdr[..., :3], o_hdr.astype(np.float32))
+    assert int(st["attempts"]) == int(ost.attempts)
 
 
 def test_reference_defaults_scene(torch_gpu):
@@ -128,6 +132,11 @@ def test_reference_defaults_scene(torch_gpu):
     assert m["class_mismatch_non_boundary"] == 0
     assert m["class_mismatch_frac"] <= FP64_MAX_MISMATCH_FRAC
     assert m["max_rel"] <= FP64_MAX_REL
+    # the as-written path (reference right-hand side, secant root, tolerance
+    # floor 0, four samples per pixel) bit for bit as well
+    assert np.array_equal(g_cls, o_cls)
+    assert np.array_equal(g_hdr[..., :3], o_hdr.astype(np.float32))
+    assert int(st["attempts"]) == int(ost.attempts)
 
 
 def test_defaults_frame_matches_reference_artefact(torch_gpu):
